@@ -1,0 +1,202 @@
+/*
+ * mista — MI355X-native STA/TFM delay-and-sum imaging, C ABI.
+ *
+ * Drop-in boundary for the reference's STA array-processor family
+ * (src/user/processor/ArrayProcessor.h:27-40 and its implementations
+ * VectorialSTAProcessor.h:58-341, VectorialSTACUDAProcessor.h:40-127,
+ * VectorialSTAOCLProcessor.h:73-680).  The reference exposes that family as a
+ * C++ class: constructor(config, acquisition, upsamplingFactor, coherenceFactor,
+ * peakOffset, rxApod) + prepare(baseElement) + process(gridData).  This header
+ * is the same contract as POD-only C functions; the C++ adapter in
+ * us_lab4a_amd/include/mista/HipSTAProcessor.h rebuilds the class on top.
+ *
+ * Conventions
+ *  - Every function returns MISTA_OK (0) or a negative MISTA_E_* status; the
+ *    message is available from mista_last_error(handle) (or NULL handle for
+ *    errors raised before a handle exists).  No exception crosses the ABI.
+ *  - Handles are single-threaded.  The library owns its device buffers and a
+ *    default-priority HIP stream per handle; it never calls hipDeviceReset.
+ *  - "_device" entry points take device pointers and an optional hipStream_t
+ *    (passed as void*, NULL = the handle's stream) and return without
+ *    synchronising.  Host entry points are synchronous.
+ *  - Grid layout follows ImageGrid.h:137-143: n1 columns (x) by n2 rows (z),
+ *    z fastest; point index = col*n2 + row.
+ *  - Echo layout follows STAAcquisition.h:71-86 per tx (Matrix[rx][samples],
+ *    row-major) stacked over tx: [ntx][nrx][ns] float32.
+ */
+#ifndef MISTA_H
+#define MISTA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MISTA_ABI_VERSION 1
+
+/* ---- status codes (Exception.h:38-94 analogues) ---- */
+#define MISTA_OK               0
+#define MISTA_E_INVALID_PARAM -1  /* InvalidParameterException                 */
+#define MISTA_E_INVALID_VALUE -2  /* InvalidValueException (e.g. dead zone)     */
+#define MISTA_E_INVALID_STATE -3  /* InvalidStateException (missing grid/apod)  */
+#define MISTA_E_HIP           -4  /* HIP runtime/library error (CUDAException)  */
+#define MISTA_E_NO_MEMORY     -5  /* device allocation failed                   */
+#define MISTA_E_BUSY          -6  /* serving engine full (admission refused)    */
+
+/* ---- coherence factor: CoherenceFactor.h:139-152 (+ SCF extension) ---- */
+#define MISTA_CF_NONE 0
+#define MISTA_CF_PCF  1   /* phase coherence factor, param = gamma              */
+#define MISTA_CF_SCF  2   /* sign coherence factor of Re(per-rx sum), param = p */
+
+/* ---- analytic-signal storage in HBM (accumulation is always FP32) ---- */
+#define MISTA_STORAGE_F32  0
+#define MISTA_STORAGE_F16  1  /* per-channel scale = max(|re|,|im|)             */
+#define MISTA_STORAGE_BF16 2  /* per-channel scale = max(|re|,|im|)             */
+
+/*
+ * Replaces STAConfiguration<float> (STAConfiguration.h:118-165) plus the
+ * non-object constructor arguments of VectorialSTAProcessor
+ * (VectorialSTAProcessor.h:61-67) and the coherence-factor selection of
+ * AnalyticSignalCoherenceFactorProcessor (CoherenceFactor.h:139-152).
+ */
+typedef struct mista_sta_config {
+	uint32_t num_elements;       /* numElements: rx count = element count        */
+	uint32_t first_tx_elem;      /* firstTxElem                                  */
+	uint32_t last_tx_elem;       /* lastTxElem (inclusive)                       */
+	uint32_t upsampling_factor;  /* 1..64 (Interpolator.h:49 maxUpFactor)        */
+	float pitch;                 /* m                                            */
+	float sampling_frequency;    /* Hz                                           */
+	float propagation_speed;     /* m/s                                          */
+	float center_frequency;      /* Hz                                           */
+	float dead_zone_m;           /* m                                            */
+	float peak_offset;           /* periods of center_frequency                  */
+	int32_t cf_kind;             /* MISTA_CF_*                                   */
+	float cf_param;              /* gamma (PCF) or p (SCF)                       */
+	int32_t storage;             /* MISTA_STORAGE_*                              */
+	int32_t device;              /* HIP device ordinal                           */
+} mista_sta_config;
+
+typedef struct mista_handle mista_handle;
+
+/* ABI version, for a caller's dlopen check. */
+int mista_abi_version(void);
+
+/* VectorialSTAProcessor constructor (VectorialSTAProcessor.h:129-149):
+ * validates the configuration, selects the device.  rxApod is set separately
+ * with mista_set_rx_apod (the reference copies it in the constructor). */
+int mista_create(const mista_sta_config* config, mista_handle** out);
+
+/* Processor destructor.  Frees device memory; never resets the device. */
+int mista_destroy(mista_handle* h);
+
+/* Last error message of `h`, or of the calling thread when h is NULL. */
+const char* mista_last_error(const mista_handle* h);
+
+/* ArrayProcessor::prepare(baseElement) (ArrayProcessor.h:33,
+ * VectorialSTAProcessor.h:151-156).  Records the base element; the echo
+ * source is the caller's (see mista_process). */
+int mista_prepare(mista_handle* h, uint32_t base_element);
+
+/* rxApod constructor argument (VectorialSTAProcessor.h:67,121): n must equal
+ * num_elements.  Copied. */
+int mista_set_rx_apod(mista_handle* h, const float* rx_apod, uint32_t n);
+
+/* Extension (no reference counterpart): per-tx apodization, n = number of tx
+ * (last-first+1); NULL restores all ones (the reference behaviour). */
+int mista_set_tx_apod(mista_handle* h, const float* tx_apod, uint32_t n);
+
+/* Image grid: grid_xz is [n1][n2][2] (x, z) in m, the x/z fields of the
+ * reference's Matrix<XYZValueFactor<float>> gridData (XYZValueFactor.h:23-32).
+ * Builds the element-to-pixel delay table in FP64 on the GPU, rounded to FP32
+ * samples, and keeps it resident (replaces calculateDelaysSTAKernel,
+ * VectorialSTACUDAProcessor.cu:30-53).  May be called again with a new shape. */
+int mista_set_grid(mista_handle* h, const float* grid_xz, uint32_t n1, uint32_t n2);
+
+/* Same, for a caller holding the reference's AoS points {x,y,z,value,factor}
+ * (5 floats per point, n1*n2 points). */
+int mista_set_grid_points(mista_handle* h, const float* points, uint32_t n1, uint32_t n2);
+
+/* VectorialSTAProcessor::process (VectorialSTAProcessor.h:158-341) for one
+ * frame from host memory: echoes [ntx][nrx][ns] float32 (the stacked
+ * acquisition.execute(tx) matrices).  Writes value = |sum|/(Ntx*Nrx) and
+ * factor (1 when no coherence factor) for every grid point, host arrays of
+ * n1*n2 floats.  factor may be NULL.  Synchronous. */
+int mista_process(mista_handle* h, const float* echoes, uint32_t num_samples,
+		float* value, float* factor);
+
+/* Same, writing into the reference's AoS points (value and factor fields of
+ * each {x,y,z,value,factor}); Util::resetValueFactor semantics. */
+int mista_process_points(mista_handle* h, const float* echoes, uint32_t num_samples, float* points);
+
+/* Device-resident frame: d_echoes [ntx][nrx][ns] float32 in HBM ->
+ * d_value/d_factor [n1*n2] in HBM (d_factor may be NULL).  Asynchronous on
+ * `stream` (NULL = handle stream). */
+int mista_process_device(mista_handle* h, const float* d_echoes, uint32_t num_samples,
+		float* d_value, float* d_factor, void* stream);
+
+/* Signal preparation only (interpolation, DC removal, analytic signal:
+ * VectorialSTAProcessor.h:169-231) into the handle's resident tensor. */
+int mista_prepare_signals_device(mista_handle* h, const float* d_echoes, uint32_t num_samples, void* stream);
+
+/* Delay-and-sum only over the handle's resident tensor. */
+int mista_das_device(mista_handle* h, float* d_value, float* d_factor, void* stream);
+
+/* Test access to the resident analytic tensor: host complex float32
+ * [ntx][nrx][L][2].  Upload converts to the storage type (with per-channel
+ * scale for F16/BF16); download converts back to float32. */
+int mista_upload_analytic(mista_handle* h, const float* analytic_ri, uint32_t signal_length);
+int mista_download_analytic(mista_handle* h, float* analytic_ri);
+
+/* Test access to the FP32 delay table [num_elements][n1*n2] (samples). */
+int mista_download_delays(mista_handle* h, float* delays);
+
+/* Sizes of the current frame state: signal length L, FFT size, grid points. */
+int mista_query(const mista_handle* h, uint32_t* signal_length, uint32_t* fft_size, uint64_t* num_points);
+
+/* Select the DAS kernel variant (0 = automatic/tuned, 1 = reference-shaped
+ * one-pixel-per-lane gather, 2 = LDS-tiled).  Tile parameters 0 = default. */
+int mista_set_kernel(mista_handle* h, int variant, int tile_param0, int tile_param1);
+
+/* Time of the last mista_das_device/mista_process_device DAS launch (ms,
+ * measured with HIP events on the launch stream); 0 if not available. */
+int mista_last_timing(const mista_handle* h, float* prep_ms, float* das_ms);
+
+/* ---- synthetic echoes (test and benchmark input) ----
+ * Point scatterers (x, z, amplitude) stamped as a Gaussian-modulated pulse
+ * exp(-a t^2) cos(2 pi fc t) at the round-trip time from element e_tx to
+ * e_rx, then white Gaussian noise of standard deviation noise_std from a
+ * counter-based generator (seed).  Elements: x_e = e*pitch - (N-1)*pitch/2,
+ * y = z = 0.  Output d_echoes [ntx][nrx][ns] float32 in HBM.
+ * Deterministic for fixed inputs.  scat_y may be NULL (2-D). */
+typedef struct mista_synth_params {
+	uint32_t num_tx;
+	uint32_t num_rx;
+	uint32_t num_samples;
+	uint32_t num_scatterers;
+	float sampling_frequency;
+	float propagation_speed;
+	float center_frequency;
+	float bandwidth;             /* fractional bandwidth at -6 dB              */
+	float noise_std;             /* absolute; 0 = no noise                     */
+	uint32_t pad_;
+	uint64_t seed;
+} mista_synth_params;
+
+int mista_synth_echoes_device(const mista_synth_params* p,
+		const float* tx_xyz /* [ntx][3] host */, const float* rx_xyz /* [nrx][3] host */,
+		const float* scat_xyza /* [ns][4] host: x,y,z,amp */,
+		float* d_echoes, void* stream);
+
+/* Sum of squares of a device float array (for SNR scaling), double result. */
+int mista_sum_squares_device(const float* d_x, uint64_t n, double* out, void* stream);
+
+/* Add N(0, noise_std^2) noise from the counter-based generator to d_x. */
+int mista_add_noise_device(float* d_x, uint64_t n, float noise_std, uint64_t seed, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MISTA_H */

@@ -1,0 +1,396 @@
+"""ctypes binding of the CPU oracle (liboracle.so) — TEST INFRASTRUCTURE ONLY.
+
+The oracle is a C++ restatement of the reference CPU STA processor
+(src/user/processor/VectorialSTAProcessor.h:158-341) and its DSP helpers.
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may use
+it, as the checker or as the reported CPU baseline.  The product library
+(us_lab4a_amd) never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+CF_NONE, CF_PCF, CF_SCF = 0, 1, 2
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class Sta2d(ctypes.Structure):
+    """oracle_sta2d: STAConfiguration.h:118-165 + VectorialSTAProcessor ctor args."""
+    _fields_ = [
+        ("num_elements", ctypes.c_uint32),
+        ("first_tx_elem", ctypes.c_uint32),
+        ("last_tx_elem", ctypes.c_uint32),
+        ("num_samples", ctypes.c_uint32),
+        ("upsampling_factor", ctypes.c_uint32),
+        ("cf_kind", ctypes.c_int32),
+        ("pitch", ctypes.c_double),
+        ("sampling_frequency", ctypes.c_double),
+        ("propagation_speed", ctypes.c_double),
+        ("center_frequency", ctypes.c_double),
+        ("dead_zone_m", ctypes.c_double),
+        ("peak_offset", ctypes.c_double),
+        ("cf_param", ctypes.c_double),
+    ]
+
+
+class Sta3d(ctypes.Structure):
+    _fields_ = [
+        ("num_tx", ctypes.c_uint32),
+        ("num_rx", ctypes.c_uint32),
+        ("num_samples", ctypes.c_uint32),
+        ("upsampling_factor", ctypes.c_uint32),
+        ("cf_kind", ctypes.c_int32),
+        ("pad_", ctypes.c_uint32),
+        ("sampling_frequency", ctypes.c_double),
+        ("propagation_speed", ctypes.c_double),
+        ("center_frequency", ctypes.c_double),
+        ("dead_zone_m", ctypes.c_double),
+        ("peak_offset", ctypes.c_double),
+        ("cf_param", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library not built: {LIB_PATH} (run `make -C oracle`)")
+        L = ctypes.CDLL(LIB_PATH)
+        u32, sz, dbl, i32 = ctypes.c_uint32, ctypes.c_size_t, ctypes.c_double, ctypes.c_int
+        L.oracle_last_error.restype = ctypes.c_char_p
+        L.oracle_kaiser_beta.restype = dbl
+        L.oracle_kaiser_beta.argtypes = [dbl]
+        L.oracle_kaiser_size.restype = u32
+        L.oracle_kaiser_size.argtypes = [dbl, dbl]
+        L.oracle_kaiser_window.argtypes = [u32, dbl, _f64p]
+        L.oracle_bessel_i0.restype = dbl
+        L.oracle_bessel_i0.argtypes = [dbl]
+        L.oracle_next_fast_even.restype = u32
+        L.oracle_next_fast_even.argtypes = [u32]
+        L.oracle_upsampling_filter_size.restype = u32
+        L.oracle_upsampling_filter_size.argtypes = [u32, dbl]
+        L.oracle_upsampling_filter_f64.argtypes = [u32, dbl, _f64p]
+        L.oracle_upsampling_filter_f32.argtypes = [u32, ctypes.c_float, _f32p]
+        L.oracle_interpolate_f64.argtypes = [_f64p, sz, u32, dbl, _f64p]
+        L.oracle_interpolate_f32.argtypes = [_f32p, sz, u32, ctypes.c_float, _f32p]
+        L.oracle_convolve_f64.argtypes = [_f64p, sz, _f64p, sz, _f64p]
+        L.oracle_fft_real_f64.argtypes = [_f64p, u32, _f64p, _f64p]
+        L.oracle_ifft_f64.argtypes = [_f64p, _f64p, u32, _f64p, _f64p]
+        L.oracle_analytic_signal_f64.argtypes = [_f64p, u32, _f64p]
+        L.oracle_analytic_signal_f32.argtypes = [_f32p, u32, _f32p]
+        L.oracle_remove_dc_f64.argtypes = [_f64p, sz, sz]
+        L.oracle_remove_dc_f32.argtypes = [_f32p, sz, sz]
+        L.oracle_std_dev_f64.restype = dbl
+        L.oracle_std_dev_f64.argtypes = [_f64p, sz]
+        L.oracle_mean_f64.restype = dbl
+        L.oracle_mean_f64.argtypes = [_f64p, sz]
+        L.oracle_pcf_f64.restype = dbl
+        L.oracle_pcf_f64.argtypes = [_f64p, u32, dbl]
+        L.oracle_pcf_f32.restype = ctypes.c_float
+        L.oracle_pcf_f32.argtypes = [_f32p, u32, ctypes.c_float]
+        L.oracle_scf_f64.restype = dbl
+        L.oracle_scf_f64.argtypes = [_f64p, u32, dbl]
+        L.oracle_scf_f32.restype = ctypes.c_float
+        L.oracle_scf_f32.argtypes = [_f32p, u32, ctypes.c_float]
+        L.oracle_window.restype = i32
+        L.oracle_window.argtypes = [ctypes.c_char_p, u32, _f64p]
+        P2 = ctypes.POINTER(Sta2d)
+        P3 = ctypes.POINTER(Sta3d)
+        opt32 = ctypes.c_void_p
+        L.oracle_prepare_f32.restype = i32
+        L.oracle_prepare_f32.argtypes = [P2, _f32p, _f32p, i32]
+        L.oracle_prepare_f64.restype = i32
+        L.oracle_prepare_f64.argtypes = [P2, _f32p, _f64p, i32]
+        L.oracle_das_f32.restype = i32
+        L.oracle_das_f32.argtypes = [P2, _f32p, u32, _f32p, u32, u32, _f32p, opt32, _f32p, _f32p, i32]
+        L.oracle_das_rxsums_f32.restype = i32
+        L.oracle_das_rxsums_f32.argtypes = [P2, _f32p, u32, _f32p, u32, u32, _f32p, opt32, _f32p, _f32p, _f32p, i32]
+        L.oracle_das_f64.restype = i32
+        L.oracle_das_f64.argtypes = [P2, _f64p, u32, _f32p, u32, u32, _f32p, opt32, _f64p, _f64p, i32]
+        L.oracle_process_f32.restype = i32
+        L.oracle_process_f32.argtypes = [P2, _f32p, _f32p, u32, u32, _f32p, opt32, _f32p, _f32p, i32,
+                                         ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+        L.oracle_prepare3d_f32.restype = i32
+        L.oracle_prepare3d_f32.argtypes = [P3, _f32p, _f32p, i32]
+        L.oracle_das3d_f32.restype = i32
+        L.oracle_das3d_f32.argtypes = [P3, _f32p, u32, _f32p, _f32p, _f32p, u32, u32, _f32p, _f32p, _f32p, i32]
+        L.oracle_das3d_f64.restype = i32
+        L.oracle_das3d_f64.argtypes = [P3, _f64p, u32, _f32p, _f32p, _f32p, u32, u32, _f32p, _f64p, _f64p, i32]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().oracle_last_error().decode())
+
+
+def _opt(a):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, np.float32)
+    return a.ctypes.data_as(ctypes.c_void_p), a
+
+
+# ---------------------------------------------------------------- helpers
+
+def kaiser_beta(tol_db):
+    return lib().oracle_kaiser_beta(float(tol_db))
+
+
+def kaiser_size(tol_db, tw):
+    return int(lib().oracle_kaiser_size(float(tol_db), float(tw)))
+
+
+def bessel_i0(x):
+    return lib().oracle_bessel_i0(float(x))
+
+
+def next_fast_even(n):
+    return int(lib().oracle_next_fast_even(int(n)))
+
+
+def upsampling_filter(up, half_tw=0.2, dtype=np.float64):
+    n = lib().oracle_upsampling_filter_size(up, half_tw)
+    if n == 0:
+        raise ValueError(lib().oracle_last_error().decode())
+    if dtype == np.float32:
+        h = np.empty(n, np.float32)
+        lib().oracle_upsampling_filter_f32(up, half_tw, h)
+    else:
+        h = np.empty(n, np.float64)
+        lib().oracle_upsampling_filter_f64(up, half_tw, h)
+    return h
+
+
+def interpolate(x, up, half_tw=0.2):
+    x = np.ascontiguousarray(x)
+    if x.dtype == np.float32:
+        out = np.empty(x.size * up, np.float32)
+        lib().oracle_interpolate_f32(x, x.size, up, half_tw, out)
+    else:
+        x = x.astype(np.float64)
+        out = np.empty(x.size * up, np.float64)
+        lib().oracle_interpolate_f64(x, x.size, up, half_tw, out)
+    return out
+
+
+def convolve(x, b):
+    x = np.ascontiguousarray(x, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    y = np.empty(x.size + b.size - 1)
+    lib().oracle_convolve_f64(x, x.size, b, b.size, y)
+    return y
+
+
+def fft_real(x):
+    x = np.ascontiguousarray(x, np.float64)
+    re, im = np.empty(x.size), np.empty(x.size)
+    lib().oracle_fft_real_f64(x, x.size, re, im)
+    return re + 1j * im
+
+
+def ifft(X):
+    re = np.ascontiguousarray(X.real, np.float64)
+    im = np.ascontiguousarray(X.imag, np.float64)
+    ore, oim = np.empty(re.size), np.empty(re.size)
+    lib().oracle_ifft_f64(re, im, re.size, ore, oim)
+    return ore + 1j * oim
+
+
+def analytic_signal(x):
+    x = np.ascontiguousarray(x)
+    if x.dtype == np.float32:
+        out = np.empty((x.size, 2), np.float32)
+        lib().oracle_analytic_signal_f32(x, x.size, out)
+    else:
+        x = x.astype(np.float64)
+        out = np.empty((x.size, 2), np.float64)
+        lib().oracle_analytic_signal_f64(x, x.size, out)
+    return out[:, 0] + 1j * out[:, 1]
+
+
+def remove_dc(x, begin):
+    x = np.array(x, copy=True)
+    if x.dtype == np.float32:
+        lib().oracle_remove_dc_f32(x, x.size, begin)
+    else:
+        x = x.astype(np.float64)
+        lib().oracle_remove_dc_f64(x, x.size, begin)
+    return x
+
+
+def std_dev(x):
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().oracle_std_dev_f64(x, x.size)
+
+
+def mean(x):
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().oracle_mean_f64(x, x.size)
+
+
+def pcf(c, gamma, dtype=np.float64):
+    ri = np.ascontiguousarray(np.stack([np.real(c), np.imag(c)], axis=-1).reshape(-1), dtype)
+    if dtype == np.float32:
+        return lib().oracle_pcf_f32(ri, len(c), gamma)
+    return lib().oracle_pcf_f64(ri, len(c), gamma)
+
+
+def scf(x, p, dtype=np.float64):
+    x = np.ascontiguousarray(x, dtype)
+    if dtype == np.float32:
+        return lib().oracle_scf_f32(x, x.size, p)
+    return lib().oracle_scf_f64(x, x.size, p)
+
+
+def window(name, n):
+    out = np.empty(n)
+    _check(lib().oracle_window(name.encode(), n, out))
+    return out
+
+
+# ---------------------------------------------------------------- STA path
+
+def make_cfg(num_elements, num_samples, upsampling_factor, pitch, fs, c, fc,
+             first_tx=0, last_tx=None, dead_zone_m=0.0, peak_offset=0.0,
+             cf_kind=CF_NONE, cf_param=1.0):
+    return Sta2d(num_elements, first_tx, num_elements - 1 if last_tx is None else last_tx,
+                 num_samples, upsampling_factor, cf_kind, pitch, fs, c, fc,
+                 dead_zone_m, peak_offset, cf_param)
+
+
+def prepare(cfg, echoes, dtype=np.float32, nthreads=0):
+    """Raw echoes [ntx][nrx][ns] float32 -> complex analytic tensor [ntx][nrx][ns*U]."""
+    echoes = np.ascontiguousarray(echoes, np.float32)
+    ntx, nrx, ns = echoes.shape
+    L = ns * cfg.upsampling_factor
+    out = np.empty((ntx, nrx, L, 2), dtype)
+    fn = lib().oracle_prepare_f32 if dtype == np.float32 else lib().oracle_prepare_f64
+    _check(fn(ctypes.byref(cfg), echoes, out, nthreads))
+    return out
+
+
+def das(cfg, analytic_ri, grid_xz, rx_apod, tx_apod=None, nthreads=0):
+    """analytic_ri [ntx][nrx][L][2] (float32 or float64); grid_xz [n1][n2][2]."""
+    analytic_ri = np.ascontiguousarray(analytic_ri)
+    L = analytic_ri.shape[2]
+    grid_xz = np.ascontiguousarray(grid_xz, np.float32)
+    n1, n2 = grid_xz.shape[:2]
+    rx_apod = np.ascontiguousarray(rx_apod, np.float32)
+    opt = _opt(tx_apod)
+    tx_ptr = opt[0] if opt else None
+    if analytic_ri.dtype == np.float32:
+        value = np.empty((n1, n2), np.float32)
+        factor = np.empty((n1, n2), np.float32)
+        _check(lib().oracle_das_f32(ctypes.byref(cfg), analytic_ri, L, grid_xz, n1, n2,
+                                    rx_apod, tx_ptr, value, factor, nthreads))
+    else:
+        value = np.empty((n1, n2), np.float64)
+        factor = np.empty((n1, n2), np.float64)
+        _check(lib().oracle_das_f64(ctypes.byref(cfg), analytic_ri, L, grid_xz, n1, n2,
+                                    rx_apod, tx_ptr, value, factor, nthreads))
+    return value, factor
+
+
+def das_rxsums(cfg, analytic_ri, grid_xz, rx_apod, tx_apod=None, nthreads=0):
+    """FP32 DAS with a coherence factor, also returning the per-rx sums
+    (complex [n1][n2][num_elements])."""
+    analytic_ri = np.ascontiguousarray(analytic_ri, np.float32)
+    L = analytic_ri.shape[2]
+    grid_xz = np.ascontiguousarray(grid_xz, np.float32)
+    n1, n2 = grid_xz.shape[:2]
+    rx_apod = np.ascontiguousarray(rx_apod, np.float32)
+    opt = _opt(tx_apod)
+    value = np.empty((n1, n2), np.float32)
+    factor = np.empty((n1, n2), np.float32)
+    sums = np.empty((n1, n2, cfg.num_elements, 2), np.float32)
+    _check(lib().oracle_das_rxsums_f32(ctypes.byref(cfg), analytic_ri, L, grid_xz, n1, n2, rx_apod,
+                                       opt[0] if opt else None, value, factor, sums, nthreads))
+    return value, factor, sums[..., 0] + 1j * sums[..., 1]
+
+
+def branch_sensitive(rx_sums, kind, rel=1e-2):
+    """Pixels whose coherence factor can change under rounding of the per-rx
+    sums: PCF when a sum lies within `rel` (of the pixel's largest per-rx sum)
+    of the negative real axis, where atan2 jumps by 2 pi; SCF when a real part
+    is within `rel` of zero (a sign decision).  The default margin covers
+    delays that differ by a few FP32 ulps (~0.003 samples at L ~ 2^15, i.e.
+    FP64-built delay tables against the reference's FP32 distance), which
+    moves a per-rx sum by up to ~1e-2 of the pixel's largest one."""
+    scale = np.abs(rx_sums).max(axis=-1, keepdims=True)
+    thr = rel * np.maximum(scale, np.finfo(np.float32).tiny)
+    nonzero = rx_sums != 0   # exact zeros (zero weights) are decided identically
+    if kind == CF_PCF:
+        hit = (np.abs(rx_sums.imag) <= thr) & (rx_sums.real < thr) & nonzero
+    else:
+        hit = (np.abs(rx_sums.real) <= thr) & nonzero
+    return hit.any(axis=-1)
+
+
+def branch_margin(rx_sums, kind):
+    """Per pixel, the smallest distance of a per-rx sum to its discontinuity,
+    relative to the pixel's largest per-rx sum (inf when none is near)."""
+    scale = np.maximum(np.abs(rx_sums).max(axis=-1), np.finfo(np.float32).tiny)
+    if kind == CF_PCF:
+        d = np.where(rx_sums.real < 0, np.abs(rx_sums.imag), np.abs(rx_sums))
+    else:
+        d = np.abs(rx_sums.real)
+    d = np.where(rx_sums != 0, d, np.inf)
+    return d.min(axis=-1) / scale
+
+
+def process(cfg, echoes, grid_xz, rx_apod, tx_apod=None, nthreads=0):
+    """Whole FP32 frame; returns (value, factor, t_prep_s, t_das_s)."""
+    echoes = np.ascontiguousarray(echoes, np.float32)
+    grid_xz = np.ascontiguousarray(grid_xz, np.float32)
+    n1, n2 = grid_xz.shape[:2]
+    rx_apod = np.ascontiguousarray(rx_apod, np.float32)
+    opt = _opt(tx_apod)
+    value = np.empty((n1, n2), np.float32)
+    factor = np.empty((n1, n2), np.float32)
+    tp, td = ctypes.c_double(), ctypes.c_double()
+    _check(lib().oracle_process_f32(ctypes.byref(cfg), echoes, grid_xz, n1, n2, rx_apod,
+                                    opt[0] if opt else None, value, factor, nthreads,
+                                    ctypes.byref(tp), ctypes.byref(td)))
+    return value, factor, tp.value, td.value
+
+
+def make_cfg3d(num_tx, num_rx, num_samples, upsampling_factor, fs, c, fc,
+               dead_zone_m=0.0, peak_offset=0.0, cf_kind=CF_NONE, cf_param=1.0):
+    return Sta3d(num_tx, num_rx, num_samples, upsampling_factor, cf_kind, 0, fs, c, fc,
+                 dead_zone_m, peak_offset, cf_param)
+
+
+def prepare3d(cfg, echoes, nthreads=0):
+    echoes = np.ascontiguousarray(echoes, np.float32)
+    ntx, nrx, ns = echoes.shape
+    out = np.empty((ntx, nrx, ns * cfg.upsampling_factor, 2), np.float32)
+    _check(lib().oracle_prepare3d_f32(ctypes.byref(cfg), echoes, out, nthreads))
+    return out
+
+
+def das3d(cfg, analytic_ri, tx_xy, rx_xy, grid_xyz, rx_apod, nthreads=0):
+    analytic_ri = np.ascontiguousarray(analytic_ri)
+    L = analytic_ri.shape[2]
+    grid_xyz = np.ascontiguousarray(grid_xyz, np.float32)
+    n1, n2 = grid_xyz.shape[:2]
+    tx_xy = np.ascontiguousarray(tx_xy, np.float32)
+    rx_xy = np.ascontiguousarray(rx_xy, np.float32)
+    rx_apod = np.ascontiguousarray(rx_apod, np.float32)
+    dt = analytic_ri.dtype
+    value = np.empty((n1, n2), dt)
+    factor = np.empty((n1, n2), dt)
+    fn = lib().oracle_das3d_f32 if dt == np.float32 else lib().oracle_das3d_f64
+    _check(fn(ctypes.byref(cfg), analytic_ri, L, tx_xy, rx_xy, grid_xyz, n1, n2, rx_apod,
+              value, factor, nthreads))
+    return value, factor

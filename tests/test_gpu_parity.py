@@ -1,0 +1,255 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the
+same seeded inputs.  Bar (FP32 storage): max |value_gpu - value_oracle| <=
+1e-4 x image peak; the CF image value*factor to the same bar.  FP16/BF16
+storage: envelope error in dB against the FP64 oracle, stated per test."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from us_lab4a_amd import imaging, workloads
+from us_lab4a_amd.sta import CoherenceFactor, HipSTAProcessor, STAConfiguration
+from us_lab4a_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def torch_cuda():
+    import torch
+    return torch
+
+
+def peak_err(a, b):
+    return float(np.abs(a.astype(np.float64) - b.astype(np.float64)).max() / np.abs(b).max())
+
+
+def check_cf_image(v, f, v_o, f_o, rx_sums, kind, tag):
+    """CF image value*factor within TOL of the value peak, except at pixels
+    where the oracle's per-rx sums sit on a coherence-factor discontinuity
+    (oracle.branch_sensitive: the atan2 cut for PCF, a zero real part for
+    SCF), where any two FP32 summation orders may disagree.  Those pixels
+    must stay rare."""
+    peak = float(np.abs(v_o).max())
+    bad = np.abs(v.astype(np.float64) * f - v_o.astype(np.float64) * f_o) > TOL * peak
+    sens = oracle.branch_sensitive(rx_sums, kind)
+    say(tag, "CF image over the bar at", int(bad.sum()), "of", bad.size, "pixels; branch-sensitive pixels",
+        int(sens.sum()), "; unexplained", int((bad & ~sens).sum()))
+    assert not np.any(bad & ~sens)
+    assert bad.mean() <= 0.01
+
+
+def make_proc(wl, storage="f32", cf=None):
+    cfgs = wl.sta_config()
+    p = HipSTAProcessor(cfgs, imaging.ArrayAcquisition(np.zeros((1, 1, 1), np.float32)), wl.upsampling,
+                        CoherenceFactor(cf or wl.cf, wl.cf_param), wl.peak_offset, wl.rx_apodization(),
+                        tx_apod=wl.tx_apodization(), storage=storage)
+    p.set_grid(wl.grid())
+    return p
+
+
+@pytest.fixture(scope="module")
+def small():
+    wl = workloads.sta2d_32_small()
+    echoes = wl.echoes_cpu()
+    return wl, echoes
+
+
+def test_delay_table_fp64_rounded_once():
+    wl = workloads.sta2d_128_pcf()
+    say("delay table", wl.name)
+    p = make_proc(wl)
+    d = p.download_delays()
+    g = wl.grid()
+    x = g["x"].reshape(-1).astype(np.float64)
+    z = g["z"].reshape(-1).astype(np.float64)
+    xe = np.arange(wl.num_elements) * np.float64(np.float32(wl.pitch)) - (wl.num_elements - 1) * np.float64(np.float32(wl.pitch)) / 2
+    inv = np.float64(np.float32(wl.fs)) * wl.upsampling / np.float64(np.float32(wl.c))
+    ref = (np.sqrt((x[None, :] - xe[:, None]) ** 2 + z[None, :] ** 2) * inv)
+    # one rounding to FP32: within half an ulp of the FP64 value (+ FP64 noise)
+    ulp = np.spacing(ref.astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(d.astype(np.float64) - ref) <= 0.5 * ulp + 1e-9)
+    p.close()
+
+
+def test_prepare_matches_oracle(small):
+    wl, echoes = small
+    say("prepare", wl.name)
+    p = make_proc(wl)
+    p.process_echoes(echoes)
+    a_gpu = p.download_analytic()
+    a32 = oracle.prepare(wl.oracle_cfg(), echoes, np.float32)
+    a64 = oracle.prepare(wl.oracle_cfg(), echoes, np.float64)
+    m = np.abs(a64).max()
+    assert np.abs(a_gpu - a64).max() / m <= 2e-6
+    assert np.abs(a32 - a64).max() / m <= 2e-6
+    p.close()
+
+
+@pytest.mark.parametrize("cf", ["none", "phase_coherence_factor", "sign_coherence_factor"])
+def test_das_matches_oracle_on_oracle_tensor(small, cf):
+    wl, echoes = small
+    say("das", wl.name, cf)
+    a32 = oracle.prepare(wl.oracle_cfg(), echoes, np.float32)
+    wl_cf = workloads.sta2d_32_small()
+    wl_cf.cf = cf
+    ocfg = wl_cf.oracle_cfg()
+    xz = imaging.grid_xz(wl_cf.grid())
+    if cf == "none":
+        v_o, f_o = oracle.das(ocfg, a32, xz, wl_cf.rx_apodization(), wl_cf.tx_apodization())
+    else:
+        v_o, f_o, sums = oracle.das_rxsums(ocfg, a32, xz, wl_cf.rx_apodization(), wl_cf.tx_apodization())
+    p = make_proc(wl_cf)
+    p.upload_analytic(a32)
+    import torch
+    n1, n2 = wl_cf.n_x, wl_cf.n_z
+    dv = torch.empty(n1 * n2, dtype=torch.float32, device="cuda")
+    df = torch.empty(n1 * n2, dtype=torch.float32, device="cuda")
+    p.das_device(dv, df)
+    torch.cuda.synchronize()
+    v = dv.cpu().numpy().reshape(n1, n2)
+    f = df.cpu().numpy().reshape(n1, n2)
+    assert peak_err(v, v_o) <= TOL
+    if cf == "sign_coherence_factor":
+        # SCF = |1 - sqrt(1 - k^2)|^p with k = sum(sign Re)/N: recover the sign
+        # imbalance |2 n_pos - N|; equal except at single sign decisions (step 2).
+        N = wl_cf.num_elements
+
+        def imbalance(fac):
+            m = np.clip(fac.astype(np.float64), 0, 1) ** (1.0 / wl_cf.cf_param)
+            return np.rint(np.sqrt(np.clip(1.0 - (1.0 - m) ** 2, 0, 1)) * N).astype(int)
+
+        d = np.abs(imbalance(f) - imbalance(f_o))
+        say(cf, "sign imbalance equal on", float((d == 0).mean()), "of pixels; max step", int(d.max()))
+        margins = oracle.branch_margin(sums, oracle.CF_SCF)[d > 0]
+        say(cf, "sign-flip pixels", int((d > 0).sum()), "; largest branch margin among them",
+            float(margins.max()) if margins.size else None)
+        assert d.max() <= 2
+        assert not np.any((d > 0) & ~oracle.branch_sensitive(sums, oracle.CF_SCF))
+        check_cf_image(v, f, v_o, f_o, sums, oracle.CF_SCF, cf)
+    elif cf != "none":
+        check_cf_image(v, f, v_o, f_o, sums, oracle.CF_PCF, cf)
+        close = np.abs(f - f_o) <= 1e-3
+        say(cf, "factor within 1e-3 on", float(close.mean()), "of pixels")
+        assert close.mean() >= 0.99
+    else:
+        assert np.all(f == 1.0)
+    p.close()
+
+
+@pytest.mark.parametrize("cf", ["none", "phase_coherence_factor"])
+def test_full_frame_matches_oracle(small, cf):
+    wl, echoes = small
+    wl.cf = cf
+    say("frame", wl.name, cf)
+    v_o, f_o, _, _ = oracle.process(wl.oracle_cfg(), echoes, imaging.grid_xz(wl.grid()), wl.rx_apodization(),
+                                     wl.tx_apodization())
+    cfgs = wl.sta_config()
+    grid = wl.grid()
+    acq = imaging.ArrayAcquisition(echoes)
+    p = HipSTAProcessor(cfgs, acq, wl.upsampling, CoherenceFactor(cf, 1.0), 0.0, wl.rx_apodization(),
+                        tx_apod=wl.tx_apodization())
+    p.prepare(0)
+    p.process(grid)
+    assert peak_err(grid["value"], v_o) <= TOL
+    if cf != "none":
+        assert peak_err(grid["value"] * grid["factor"], v_o * f_o) <= TOL
+    # determinism: a second frame is bit-identical
+    g2 = wl.grid()
+    p.process(g2)
+    assert np.array_equal(g2["value"], grid["value"]) and np.array_equal(g2["factor"], grid["factor"])
+    p.close()
+    wl.cf = "none"
+
+
+@pytest.mark.parametrize("storage,limit_db", [("f16", -60.0), ("bf16", -40.0)])
+def test_reduced_storage_db_error_vs_fp64(small, storage, limit_db):
+    wl, echoes = small
+    say("storage", storage)
+    a64 = oracle.prepare(wl.oracle_cfg(), echoes, np.float64)
+    xz = imaging.grid_xz(wl.grid())
+    v64, _ = oracle.das(wl.oracle_cfg(), a64, xz, wl.rx_apodization(), wl.tx_apodization())
+    p = make_proc(wl, storage=storage)
+    v, _ = p.process_echoes(echoes)
+    err = workloads.db_error(v, v64)
+    say(storage, "envelope error vs FP64 (dB):", round(err, 1))
+    assert err <= limit_db
+    p32 = make_proc(wl)
+    v32, _ = p32.process_echoes(echoes)
+    err32 = workloads.db_error(v32, v64)
+    say("f32 envelope error vs FP64 (dB):", round(err32, 1))
+    assert err32 <= -80.0
+    p.close()
+    p32.close()
+
+
+def test_graded_shape_columns_match_oracle():
+    """128 elements, 128x128x4096 echoes (GPU-synthesised), U = 8, PCF, grid
+    512x1024: prep on a channel subset and DAS on a column subset against the
+    oracle, with the full frame run on the GPU."""
+    import torch
+    wl = workloads.sta2d_128_pcf()
+    say("graded shape", wl.name)
+    d_echoes = torch.empty((wl.num_tx, wl.num_elements, wl.num_samples), dtype=torch.float32, device="cuda")
+    wl.echoes_gpu(d_echoes)
+    torch.cuda.synchronize()
+    p = make_proc(wl)
+    npix = wl.n_x * wl.n_z
+    dv = torch.empty(npix, dtype=torch.float32, device="cuda")
+    df = torch.empty(npix, dtype=torch.float32, device="cuda")
+    p.process_device(d_echoes, wl.num_samples, dv, df)
+    torch.cuda.synchronize()
+    say("graded shape: frame done")
+    v = dv.cpu().numpy().reshape(wl.n_x, wl.n_z)
+    f = df.cpu().numpy().reshape(wl.n_x, wl.n_z)
+    assert np.isfinite(v).all() and (f >= 0).all() and (f <= 1).all()
+    echoes = d_echoes.cpu().numpy()
+    a_gpu = p.download_analytic()
+    say("graded shape: tensor downloaded")
+    # prep parity on a channel subset (oracle FP64)
+    sub = echoes[[0, 63, 127]][:, [0, 64, 127]]
+    cfg_sub = wl.oracle_cfg()
+    cfg_sub.num_elements = 3
+    cfg_sub.last_tx_elem = 2
+    a_sub = oracle.prepare(cfg_sub, sub, np.float64)
+    ref = a_gpu[[0, 63, 127]][:, [0, 64, 127]]
+    assert np.abs(ref - a_sub).max() / np.abs(a_sub).max() <= 2e-6
+    # DAS parity on a column subset, oracle FP32 over the GPU tensor
+    cols = [0, 137, 255, 256, 400, 511]
+    xz = imaging.grid_xz(wl.grid())[cols]
+    v_o, f_o, sums = oracle.das_rxsums(wl.oracle_cfg(), a_gpu, np.ascontiguousarray(xz), wl.rx_apodization())
+    peak = np.abs(v).max()
+    assert np.abs(v[cols] - v_o).max() / peak <= TOL
+    # CF image against the full-image peak (the column subset may miss it)
+    bad = np.abs(v[cols].astype(np.float64) * f[cols] - v_o.astype(np.float64) * f_o) > TOL * peak
+    sens = oracle.branch_sensitive(sums, oracle.CF_PCF)
+    say("graded shape: CF image over the bar at", int(bad.sum()), "pixels; unexplained", int((bad & ~sens).sum()),
+        "; branch margins of the over-bar pixels", oracle.branch_margin(sums, oracle.CF_PCF)[bad].tolist())
+    assert not np.any(bad & ~sens) and bad.mean() <= 0.01
+    p.close()
+
+
+def test_error_paths_raise_reference_style_errors():
+    wl = workloads.tiny()
+    say("error paths")
+    cfgs = wl.sta_config()
+    p = HipSTAProcessor(cfgs, None, wl.upsampling, CoherenceFactor(), 0.0, wl.rx_apodization())
+    with pytest.raises(_lib.InvalidStateError):
+        p.process_echoes(np.zeros((wl.num_tx, wl.num_elements, wl.num_samples), np.float32))
+    with pytest.raises(_lib.InvalidValueError):
+        p.set_rx_apod(np.ones(wl.num_elements + 1, np.float32))
+    p.close()
+    cfgs.dead_zone_m = 1.0   # 1 m dead zone >= record length
+    p = HipSTAProcessor(cfgs, None, wl.upsampling, CoherenceFactor(), 0.0, wl.rx_apodization())
+    p.set_grid(wl.grid())
+    with pytest.raises(_lib.InvalidValueError, match="dead zone"):
+        p.process_echoes(np.zeros((wl.num_tx, wl.num_elements, wl.num_samples), np.float32))
+    p.close()

@@ -1,0 +1,98 @@
+"""Build the mista HIP library (gfx950) and the C++ host driver in-tree.
+
+    python -m us_lab4a_amd.build          # library + host driver
+    python -m us_lab4a_amd.build --clean
+
+The library is a single shared object, us_lab4a_amd/libmista_hip.so, built by
+hipcc for gfx950 only.  Packed-FP32 instruction selection is disabled for the
+device code (-Xclang -target-feature -Xclang -packed-fp32-ops); hipcc's host
+pass warns that it does not know the feature, which is harmless.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "libmista_hip.so")
+HOST_DRIVER = os.path.join(PKG, "mista_sta_cli")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+SOURCES = ["api.cpp", "prep.hip", "das.hip", "synth.hip", "serve.cpp"]
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd))
+    # keep genuine warnings visible, drop the known host-pass feature notice
+    noise = "'-packed-fp32-ops' is not a recognized feature for this target"
+    for line in (r.stdout + r.stderr).splitlines():
+        if noise not in line and line.strip():
+            sys.stderr.write(line + "\n")
+
+
+def build_library(verbose=True):
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    build_dir = os.path.join(PKG, "_build")
+    os.makedirs(build_dir, exist_ok=True)
+    objs = []
+    for src in srcs:
+        obj = os.path.join(build_dir, os.path.basename(src) + ".o")
+        lang = ["-x", "hip"]
+        cmd = [HIPCC] + HIP_FLAGS + lang + ["-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
+        if verbose:
+            print("[build]", os.path.relpath(src, ROOT), flush=True)
+        _run(cmd)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_host_driver(verbose=True):
+    """C++ host program above the C ABI (mirror of STAMethod's call sequence)."""
+    src = os.path.join(PKG, "host", "sta_cli.cpp")
+    if not os.path.exists(src):
+        return None
+    if verbose:
+        print("[build]", os.path.relpath(src, ROOT), flush=True)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+          "-I", os.path.join(PKG, "include"), src, "-o", HOST_DRIVER,
+          "-L", PKG, "-lmista_hip", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN"])
+    return HOST_DRIVER
+
+
+def build_oracle(verbose=True):
+    """Test-infrastructure oracle (CPU), oracle/liboracle.so."""
+    odir = os.path.join(ROOT, "oracle")
+    if verbose:
+        print("[build] oracle/liboracle.so", flush=True)
+    _run(["make", "-s", "-C", odir, "-B", "liboracle.so"])
+
+
+def clean():
+    shutil.rmtree(os.path.join(PKG, "_build"), ignore_errors=True)
+    for p in (LIB, HOST_DRIVER):
+        if os.path.exists(p):
+            os.remove(p)
+
+
+if __name__ == "__main__":
+    if "--clean" in sys.argv:
+        clean()
+    else:
+        build_library()
+        build_host_driver()
+        build_oracle()

@@ -1,0 +1,525 @@
+// C ABI of the mista library (include/mista.h): the STA array-processor
+// contract of the reference (ArrayProcessor.h:27-40,
+// VectorialSTAProcessor.h:58-341) over HIP device buffers.
+#include "common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace mista {
+void synth_echoes(const mista_synth_params& p, const float* tx_xyz, const float* rx_xyz, const float* scat,
+		float* d_echoes, hipStream_t s);
+double sum_squares(const float* d_x, uint64_t n, hipStream_t s);
+void add_noise(float* d_x, uint64_t n, float sigma, uint64_t seed, hipStream_t s);
+}  // namespace mista
+
+using namespace mista;
+
+namespace {
+thread_local std::string g_thread_error;
+}
+
+struct mista_handle {
+	mista_sta_config cfg{};
+	hipStream_t stream = nullptr;
+	std::string last_error;
+	uint32_t base_element = 0;
+
+	// apodization (host copies + device copies)
+	std::vector<float> rx_apod, tx_apod;
+	bool tx_apod_set = false;
+	DeviceBuffer d_rx_apod, d_tx_apod, d_w;
+	bool weights_dirty = true;
+
+	// grid + resident delay table
+	uint32_t n1 = 0, n2 = 0;
+	DeviceBuffer d_grid, d_delay;
+
+	// frame state
+	FrameGeom geom{};
+	bool signal_ready = false;     // tensor holds a prepared frame
+	DeviceBuffer d_raw, d_tensor, d_scale, d_hpoly, d_tmp, d_value, d_factor;
+	uint32_t filter_J = 0, filter_off = 0, filter_up = 0;
+	PrepState prep;
+	DasTuning tuning;
+	DasScratch das_scratch;
+
+	// timing
+	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+	bool timed = false;
+
+	~mista_handle()
+	{
+		for (auto& e : ev)
+			if (e) (void)hipEventDestroy(e);
+		if (stream) (void)hipStreamDestroy(stream);
+	}
+};
+
+namespace {
+
+template<typename F>
+int guarded(mista_handle* h, F&& f)
+{
+	try {
+		if (h) HIP_CHECK(hipSetDevice(h->cfg.device));
+		f();
+		return MISTA_OK;
+	} catch (const Error& e) {
+		if (h) h->last_error = e.what();
+		g_thread_error = e.what();
+		return e.code;
+	} catch (const std::bad_alloc&) {
+		if (h) h->last_error = "host allocation failed";
+		g_thread_error = "host allocation failed";
+		return MISTA_E_NO_MEMORY;
+	} catch (const std::exception& e) {
+		if (h) h->last_error = e.what();
+		g_thread_error = e.what();
+		return MISTA_E_INVALID_STATE;
+	}
+}
+
+hipStream_t pick_stream(mista_handle* h, void* stream)
+{
+	return stream ? static_cast<hipStream_t>(stream) : h->stream;
+}
+
+void validate(const mista_sta_config& c)
+{
+	if (c.num_elements == 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "The number of elements is zero.");
+	if (c.last_tx_elem < c.first_tx_elem || c.last_tx_elem >= c.num_elements)
+		MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid tx element range [" << c.first_tx_elem << ", " << c.last_tx_elem
+				<< "] for " << c.num_elements << " elements.");
+	if (c.upsampling_factor < 1 || c.upsampling_factor > 64)
+		MISTA_THROW(MISTA_E_INVALID_VALUE, "Invalid upsampling factor: " << c.upsampling_factor << ". Must be in [1, 64].");
+	if (!(c.sampling_frequency > 0) || !(c.propagation_speed > 0) || !(c.center_frequency > 0))
+		MISTA_THROW(MISTA_E_INVALID_PARAM, "sampling_frequency, propagation_speed and center_frequency must be > 0.");
+	if (!(c.pitch > 0)) MISTA_THROW(MISTA_E_INVALID_PARAM, "pitch must be > 0.");
+	if (!(c.dead_zone_m >= 0)) MISTA_THROW(MISTA_E_INVALID_PARAM, "dead_zone_m must be >= 0.");
+	if (!(c.peak_offset >= 0)) MISTA_THROW(MISTA_E_INVALID_PARAM, "peak_offset must be >= 0.");
+	if (c.cf_kind != MISTA_CF_NONE && c.cf_kind != MISTA_CF_PCF && c.cf_kind != MISTA_CF_SCF)
+		MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid coherence factor method: " << c.cf_kind << '.');
+	if (c.cf_kind != MISTA_CF_NONE && !(c.cf_param >= 0 && c.cf_param <= 100))
+		MISTA_THROW(MISTA_E_INVALID_PARAM, "coherence factor parameter out of [0, 100].");
+	if (c.storage != MISTA_STORAGE_F32 && c.storage != MISTA_STORAGE_F16 && c.storage != MISTA_STORAGE_BF16)
+		MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid storage type: " << c.storage << '.');
+}
+
+uint32_t ntx_of(const mista_sta_config& c) { return c.last_tx_elem - c.first_tx_elem + 1U; }
+
+// Dead zone in upsampled samples, VectorialSTAProcessor.h:138 (float
+// configuration: fs*U in float, the rest in double).
+uint32_t dead_zone_up(const mista_sta_config& c)
+{
+	const float fsu = static_cast<float>(c.upsampling_factor) * c.sampling_frequency;
+	return static_cast<uint32_t>(fsu * 2.0 * static_cast<double>(c.dead_zone_m) / static_cast<double>(c.propagation_speed));
+}
+
+// Shapes of a frame with signal length L (ns raw samples; ns == 0 when the
+// tensor is uploaded directly).
+void set_frame_geom(mista_handle* h, uint32_t ns, uint32_t L)
+{
+	const mista_sta_config& c = h->cfg;
+	FrameGeom& g = h->geom;
+	g.num_elements = c.num_elements;
+	g.first_tx = c.first_tx_elem;
+	g.ntx = ntx_of(c);
+	g.nrx = c.num_elements;
+	g.ns = ns;
+	g.up = c.upsampling_factor;
+	g.L = L;
+	g.dz_up = dead_zone_up(c);
+	if (g.dz_up >= L)
+		MISTA_THROW(MISTA_E_INVALID_VALUE, "Invalid dead zone: deadZoneSamplesUp (" << g.dz_up << ") >= signalLength (" << L << ").");
+	if (L < 2) MISTA_THROW(MISTA_E_INVALID_PARAM, "signal length must be >= 2.");
+	g.nf = hilbert_fft_size(L);
+	g.n1 = h->n1;
+	g.n2 = h->n2;
+	g.npix = static_cast<uint64_t>(h->n1) * h->n2;
+	// VectorialSTAProcessor.h:147 (float): (fs*U) * peakOffset / fc
+	g.offset = (c.sampling_frequency * static_cast<float>(c.upsampling_factor)) * c.peak_offset / c.center_frequency;
+	g.inv_ct = (static_cast<double>(c.sampling_frequency) * c.upsampling_factor) / static_cast<double>(c.propagation_speed);
+	g.pitch = c.pitch;
+	g.cf_kind = c.cf_kind;
+	g.cf_param = c.cf_param;
+	g.storage = c.storage;
+	const size_t nch = static_cast<size_t>(g.ntx) * g.nrx;
+	h->d_tensor.reserve(nch * L * storage_bytes(c.storage));
+	h->d_scale.reserve(nch * sizeof(float));
+	h->d_w.reserve(nch * sizeof(float));
+}
+
+void ensure_filter(mista_handle* h, hipStream_t s)
+{
+	const uint32_t up = h->cfg.upsampling_factor;
+	if (up < 2 || h->filter_up == up) return;
+	// VectorialSTAProcessor.h:92 upsampFilterHalfTransitionWidth = 0.2
+	const std::vector<double> taps = upsampling_filter(up, 0.2);
+	const uint32_t T = static_cast<uint32_t>(taps.size());
+	const uint32_t J = (T + up - 1) / up;
+	std::vector<float> poly(static_cast<size_t>(up) * J, 0.0f);
+	for (uint32_t ph = 0; ph < up; ++ph)
+		for (uint32_t j = 0; j < J; ++j)
+			if (ph + up * j < T) poly[static_cast<size_t>(ph) * J + j] = static_cast<float>(taps[ph + up * j]);
+	void* d = h->d_hpoly.reserve(poly.size() * sizeof(float));
+	HIP_CHECK(hipMemcpyAsync(d, poly.data(), poly.size() * sizeof(float), hipMemcpyHostToDevice, s));
+	HIP_CHECK(hipStreamSynchronize(s));
+	h->filter_J = J;
+	h->filter_off = (T - 1) / 2;
+	h->filter_up = up;
+}
+
+void require_grid(mista_handle* h)
+{
+	if (h->n1 == 0 || h->n2 == 0) MISTA_THROW(MISTA_E_INVALID_STATE, "The image grid has not been set.");
+}
+
+void upload_apod(mista_handle* h, hipStream_t s)
+{
+	if (!h->weights_dirty) return;
+	const size_t N = h->cfg.num_elements;
+	HIP_CHECK(hipMemcpyAsync(h->d_rx_apod.reserve(N * sizeof(float)), h->rx_apod.data(), N * sizeof(float),
+			hipMemcpyHostToDevice, s));
+	if (h->tx_apod_set) {
+		const size_t T = h->tx_apod.size();
+		HIP_CHECK(hipMemcpyAsync(h->d_tx_apod.reserve(T * sizeof(float)), h->tx_apod.data(), T * sizeof(float),
+				hipMemcpyHostToDevice, s));
+	}
+	HIP_CHECK(hipStreamSynchronize(s));
+	h->weights_dirty = false;
+}
+
+void ensure_events(mista_handle* h)
+{
+	for (auto& e : h->ev)
+		if (!e) HIP_CHECK(hipEventCreate(&e));
+}
+
+void run_prepare(mista_handle* h, const float* d_echoes, uint32_t ns, hipStream_t s)
+{
+	require_grid(h);
+	if (ns == 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "The number of input samples is 0.");
+	const uint64_t L64 = static_cast<uint64_t>(ns) * h->cfg.upsampling_factor;
+	if (L64 > (1u << 20)) MISTA_THROW(MISTA_E_INVALID_PARAM, "The number of input samples (" << L64 << ") is greater than 1048576.");
+	set_frame_geom(h, ns, static_cast<uint32_t>(L64));
+	ensure_filter(h, s);
+	ensure_events(h);
+	HIP_CHECK(hipEventRecord(h->ev[0], s));
+	prepare_signals(h->prep, h->geom, d_echoes, h->d_hpoly.as<const float>(), h->filter_J, h->filter_off,
+			h->d_tensor.get(), h->d_scale.as<float>(), s);
+	HIP_CHECK(hipEventRecord(h->ev[1], s));
+	h->signal_ready = true;
+}
+
+void run_das(mista_handle* h, float* d_value, float* d_factor, hipStream_t s)
+{
+	require_grid(h);
+	if (!h->signal_ready) MISTA_THROW(MISTA_E_INVALID_STATE, "No prepared signals: call a prepare/process/upload entry point first.");
+	if (h->geom.npix != static_cast<uint64_t>(h->n1) * h->n2) {
+		h->geom.n1 = h->n1;
+		h->geom.n2 = h->n2;
+		h->geom.npix = static_cast<uint64_t>(h->n1) * h->n2;
+	}
+	upload_apod(h, s);
+	ensure_events(h);
+	// w[tx][rx] = rxApod * txApod * per-channel storage scale
+	launch_weights(h->geom, h->d_rx_apod.as<const float>(), h->tx_apod_set ? h->d_tx_apod.as<const float>() : nullptr,
+			h->d_scale.as<const float>(), h->d_w.as<float>(), s);
+	HIP_CHECK(hipEventRecord(h->ev[2], s));
+	launch_das(h->geom, h->tuning, h->das_scratch, h->d_tensor.get(), h->d_delay.as<const float>(), h->d_w.as<const float>(),
+			d_value, d_factor, s);
+	HIP_CHECK(hipEventRecord(h->ev[3], s));
+	h->timed = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mista_abi_version(void) { return MISTA_ABI_VERSION; }
+
+const char* mista_last_error(const mista_handle* h)
+{
+	return h ? h->last_error.c_str() : g_thread_error.c_str();
+}
+
+int mista_create(const mista_sta_config* config, mista_handle** out)
+{
+	if (!config || !out) {
+		g_thread_error = "null argument";
+		return MISTA_E_INVALID_PARAM;
+	}
+	*out = nullptr;
+	return guarded(nullptr, [&] {
+		validate(*config);
+		int ndev = 0;
+		HIP_CHECK(hipGetDeviceCount(&ndev));
+		if (config->device < 0 || config->device >= ndev)
+			MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid device " << config->device << " (" << ndev << " devices).");
+		HIP_CHECK(hipSetDevice(config->device));
+		auto h = std::make_unique<mista_handle>();
+		h->cfg = *config;
+		HIP_CHECK(hipStreamCreate(&h->stream));  // default priority
+		h->rx_apod.assign(config->num_elements, 1.0f);
+		*out = h.release();
+	});
+}
+
+int mista_destroy(mista_handle* h)
+{
+	if (!h) return MISTA_OK;
+	(void)hipSetDevice(h->cfg.device);
+	if (h->stream) (void)hipStreamSynchronize(h->stream);
+	delete h;
+	return MISTA_OK;
+}
+
+int mista_prepare(mista_handle* h, uint32_t base_element)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	h->base_element = base_element;
+	return MISTA_OK;
+}
+
+int mista_set_rx_apod(mista_handle* h, const float* rx_apod, uint32_t n)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!rx_apod) MISTA_THROW(MISTA_E_INVALID_PARAM, "rx_apod is null.");
+		if (n != h->cfg.num_elements)
+			MISTA_THROW(MISTA_E_INVALID_VALUE, "Wrong rx apodization size: " << n << " (should be " << h->cfg.num_elements << ").");
+		h->rx_apod.assign(rx_apod, rx_apod + n);
+		h->weights_dirty = true;
+	});
+}
+
+int mista_set_tx_apod(mista_handle* h, const float* tx_apod, uint32_t n)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!tx_apod) {
+			h->tx_apod.clear();
+			h->tx_apod_set = false;
+		} else {
+			if (n != ntx_of(h->cfg))
+				MISTA_THROW(MISTA_E_INVALID_VALUE, "Wrong tx apodization size: " << n << " (should be " << ntx_of(h->cfg) << ").");
+			h->tx_apod.assign(tx_apod, tx_apod + n);
+			h->tx_apod_set = true;
+		}
+		h->weights_dirty = true;
+	});
+}
+
+int mista_set_grid(mista_handle* h, const float* grid_xz, uint32_t n1, uint32_t n2)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!grid_xz || n1 == 0 || n2 == 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "Empty image grid.");
+		const uint64_t npix = static_cast<uint64_t>(n1) * n2;
+		hipStream_t s = h->stream;
+		void* d_grid = h->d_grid.reserve(npix * 2 * sizeof(float));
+		HIP_CHECK(hipMemcpyAsync(d_grid, grid_xz, npix * 2 * sizeof(float), hipMemcpyHostToDevice, s));
+		float* d_delay = static_cast<float*>(h->d_delay.reserve(npix * h->cfg.num_elements * sizeof(float)));
+		const double inv_ct = (static_cast<double>(h->cfg.sampling_frequency) * h->cfg.upsampling_factor) /
+				static_cast<double>(h->cfg.propagation_speed);
+		launch_delay_table(static_cast<const float*>(d_grid), npix, h->cfg.num_elements, static_cast<double>(h->cfg.pitch),
+				inv_ct, d_delay, s);
+		HIP_CHECK(hipStreamSynchronize(s));
+		h->n1 = n1;
+		h->n2 = n2;
+	});
+}
+
+int mista_set_grid_points(mista_handle* h, const float* points, uint32_t n1, uint32_t n2)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	if (!points) return guarded(h, [&] { MISTA_THROW(MISTA_E_INVALID_PARAM, "points is null."); });
+	const uint64_t npix = static_cast<uint64_t>(n1) * n2;
+	std::vector<float> xz(npix * 2);
+	for (uint64_t i = 0; i < npix; ++i) {
+		xz[2 * i] = points[5 * i];
+		xz[2 * i + 1] = points[5 * i + 2];
+	}
+	return mista_set_grid(h, xz.data(), n1, n2);
+}
+
+int mista_process_device(mista_handle* h, const float* d_echoes, uint32_t num_samples, float* d_value, float* d_factor,
+		void* stream)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!d_echoes || !d_value) MISTA_THROW(MISTA_E_INVALID_PARAM, "null device pointer.");
+		hipStream_t s = pick_stream(h, stream);
+		run_prepare(h, d_echoes, num_samples, s);
+		run_das(h, d_value, d_factor, s);
+	});
+}
+
+int mista_prepare_signals_device(mista_handle* h, const float* d_echoes, uint32_t num_samples, void* stream)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!d_echoes) MISTA_THROW(MISTA_E_INVALID_PARAM, "null device pointer.");
+		run_prepare(h, d_echoes, num_samples, pick_stream(h, stream));
+	});
+}
+
+int mista_das_device(mista_handle* h, float* d_value, float* d_factor, void* stream)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!d_value) MISTA_THROW(MISTA_E_INVALID_PARAM, "null device pointer.");
+		run_das(h, d_value, d_factor, pick_stream(h, stream));
+	});
+}
+
+int mista_process(mista_handle* h, const float* echoes, uint32_t num_samples, float* value, float* factor)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!echoes || !value) MISTA_THROW(MISTA_E_INVALID_PARAM, "null host pointer.");
+		require_grid(h);
+		hipStream_t s = h->stream;
+		const uint64_t n = static_cast<uint64_t>(ntx_of(h->cfg)) * h->cfg.num_elements * num_samples;
+		float* d_raw = static_cast<float*>(h->d_raw.reserve(n * sizeof(float)));
+		HIP_CHECK(hipMemcpyAsync(d_raw, echoes, n * sizeof(float), hipMemcpyHostToDevice, s));
+		const uint64_t npix = static_cast<uint64_t>(h->n1) * h->n2;
+		float* d_value = static_cast<float*>(h->d_value.reserve(npix * sizeof(float)));
+		float* d_factor = static_cast<float*>(h->d_factor.reserve(npix * sizeof(float)));
+		run_prepare(h, d_raw, num_samples, s);
+		run_das(h, d_value, d_factor, s);
+		HIP_CHECK(hipMemcpyAsync(value, d_value, npix * sizeof(float), hipMemcpyDeviceToHost, s));
+		if (factor) HIP_CHECK(hipMemcpyAsync(factor, d_factor, npix * sizeof(float), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+	});
+}
+
+int mista_process_points(mista_handle* h, const float* echoes, uint32_t num_samples, float* points)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	if (!points) return guarded(h, [&] { MISTA_THROW(MISTA_E_INVALID_PARAM, "points is null."); });
+	const uint64_t npix = static_cast<uint64_t>(h->n1) * h->n2;
+	std::vector<float> value(npix), factor(npix);
+	const int rc = mista_process(h, echoes, num_samples, value.data(), factor.data());
+	if (rc != MISTA_OK) return rc;
+	for (uint64_t i = 0; i < npix; ++i) {
+		points[5 * i + 3] = value[i];
+		points[5 * i + 4] = factor[i];
+	}
+	return MISTA_OK;
+}
+
+int mista_upload_analytic(mista_handle* h, const float* analytic_ri, uint32_t signal_length)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!analytic_ri) MISTA_THROW(MISTA_E_INVALID_PARAM, "null host pointer.");
+		require_grid(h);
+		hipStream_t s = h->stream;
+		set_frame_geom(h, 0, signal_length);
+		const size_t nch = static_cast<size_t>(h->geom.ntx) * h->geom.nrx;
+		const size_t bytes = nch * signal_length * 2 * sizeof(float);
+		float* d_tmp = static_cast<float*>(h->d_tmp.reserve(bytes));
+		HIP_CHECK(hipMemcpyAsync(d_tmp, analytic_ri, bytes, hipMemcpyHostToDevice, s));
+		launch_tensor_convert_in(h->geom, d_tmp, h->d_tensor.get(), h->d_scale.as<float>(), s);
+		HIP_CHECK(hipStreamSynchronize(s));
+		h->d_tmp.release();
+		h->signal_ready = true;
+	});
+}
+
+int mista_download_analytic(mista_handle* h, float* analytic_ri)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!analytic_ri) MISTA_THROW(MISTA_E_INVALID_PARAM, "null host pointer.");
+		if (!h->signal_ready) MISTA_THROW(MISTA_E_INVALID_STATE, "No prepared signals.");
+		hipStream_t s = h->stream;
+		const size_t nch = static_cast<size_t>(h->geom.ntx) * h->geom.nrx;
+		const size_t bytes = nch * h->geom.L * 2 * sizeof(float);
+		float* d_tmp = static_cast<float*>(h->d_tmp.reserve(bytes));
+		launch_tensor_convert_out(h->geom, h->d_tensor.get(), h->d_scale.as<const float>(), d_tmp, s);
+		HIP_CHECK(hipMemcpyAsync(analytic_ri, d_tmp, bytes, hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		h->d_tmp.release();
+	});
+}
+
+int mista_download_delays(mista_handle* h, float* delays)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		require_grid(h);
+		if (!delays) MISTA_THROW(MISTA_E_INVALID_PARAM, "null host pointer.");
+		const size_t bytes = static_cast<size_t>(h->n1) * h->n2 * h->cfg.num_elements * sizeof(float);
+		HIP_CHECK(hipMemcpyAsync(delays, h->d_delay.get(), bytes, hipMemcpyDeviceToHost, h->stream));
+		HIP_CHECK(hipStreamSynchronize(h->stream));
+	});
+}
+
+int mista_query(const mista_handle* h, uint32_t* signal_length, uint32_t* fft_size, uint64_t* num_points)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	if (signal_length) *signal_length = h->geom.L;
+	if (fft_size) *fft_size = h->geom.nf;
+	if (num_points) *num_points = static_cast<uint64_t>(h->n1) * h->n2;
+	return MISTA_OK;
+}
+
+int mista_set_kernel(mista_handle* h, int variant, int p0, int p1)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (variant < 0 || variant > 2) MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid kernel variant " << variant);
+		h->tuning.variant = variant;
+		h->tuning.p0 = p0;
+		h->tuning.p1 = p1;
+	});
+}
+
+int mista_last_timing(const mista_handle* h, float* prep_ms, float* das_ms)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	mista_handle* hh = const_cast<mista_handle*>(h);
+	return guarded(hh, [&] {
+		float p = 0.0f, d = 0.0f;
+		if (h->timed) {
+			HIP_CHECK(hipEventSynchronize(h->ev[3]));
+			if (h->ev[0] && h->ev[1] && hipEventQuery(h->ev[1]) == hipSuccess &&
+					hipEventElapsedTime(&p, h->ev[0], h->ev[1]) != hipSuccess) p = 0.0f;
+			HIP_CHECK(hipEventElapsedTime(&d, h->ev[2], h->ev[3]));
+		}
+		if (prep_ms) *prep_ms = p;
+		if (das_ms) *das_ms = d;
+	});
+}
+
+int mista_synth_echoes_device(const mista_synth_params* p, const float* tx_xyz, const float* rx_xyz,
+		const float* scat_xyza, float* d_echoes, void* stream)
+{
+	return guarded(nullptr, [&] {
+		if (!p || !tx_xyz || !rx_xyz || !d_echoes || (p->num_scatterers && !scat_xyza))
+			MISTA_THROW(MISTA_E_INVALID_PARAM, "null argument.");
+		synth_echoes(*p, tx_xyz, rx_xyz, scat_xyza, d_echoes, static_cast<hipStream_t>(stream));
+	});
+}
+
+int mista_sum_squares_device(const float* d_x, uint64_t n, double* out, void* stream)
+{
+	return guarded(nullptr, [&] {
+		if (!d_x || !out) MISTA_THROW(MISTA_E_INVALID_PARAM, "null argument.");
+		*out = sum_squares(d_x, n, static_cast<hipStream_t>(stream));
+	});
+}
+
+int mista_add_noise_device(float* d_x, uint64_t n, float noise_std, uint64_t seed, void* stream)
+{
+	return guarded(nullptr, [&] {
+		if (!d_x) MISTA_THROW(MISTA_E_INVALID_PARAM, "null argument.");
+		add_noise(d_x, n, noise_std, seed, static_cast<hipStream_t>(stream));
+	});
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+// Internal declarations shared by the mista HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hipfft/hipfft.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/mista.h"
+
+namespace mista {
+
+// Error carried to the C ABI as a status code + message.
+struct Error : std::runtime_error {
+	int code;
+	Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define MISTA_THROW(code, msg) do { std::ostringstream os_; os_ << msg; throw ::mista::Error(code, os_.str()); } while (0)
+
+#define HIP_CHECK(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) \
+	MISTA_THROW(MISTA_E_HIP, #expr << ": " << hipGetErrorString(e_) << " (" << __FILE__ << ":" << __LINE__ << ")"); } while (0)
+
+#define FFT_CHECK(expr) do { hipfftResult r_ = (expr); if (r_ != HIPFFT_SUCCESS) \
+	MISTA_THROW(MISTA_E_HIP, #expr << ": hipfft error " << static_cast<int>(r_) << " (" << __FILE__ << ":" << __LINE__ << ")"); } while (0)
+
+// Device buffer owned by a handle.  Reallocates only when growing.
+class DeviceBuffer {
+public:
+	DeviceBuffer() = default;
+	~DeviceBuffer() { release(); }
+	DeviceBuffer(const DeviceBuffer&) = delete;
+	DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+	void* reserve(size_t bytes)
+	{
+		if (bytes > cap_) {
+			release();
+			if (bytes == 0) return nullptr;
+			hipError_t e = hipMalloc(&ptr_, bytes);
+			if (e != hipSuccess) {
+				ptr_ = nullptr;
+				MISTA_THROW(MISTA_E_NO_MEMORY, "hipMalloc(" << bytes << " bytes) failed: " << hipGetErrorString(e));
+			}
+			cap_ = bytes;
+		}
+		return ptr_;
+	}
+	void release()
+	{
+		if (ptr_) (void)hipFree(ptr_);
+		ptr_ = nullptr;
+		cap_ = 0;
+	}
+	void* get() const { return ptr_; }
+	size_t capacity() const { return cap_; }
+	template<typename T> T* as() const { return static_cast<T*>(ptr_); }
+private:
+	void* ptr_ = nullptr;
+	size_t cap_ = 0;
+};
+
+// Geometry of one 2-D frame, everything the kernels need by value.
+struct FrameGeom {
+	uint32_t num_elements;
+	uint32_t first_tx;
+	uint32_t ntx;
+	uint32_t nrx;
+	uint32_t ns;        // raw samples per channel
+	uint32_t up;        // upsampling factor
+	uint32_t L;         // signal length = ns * up
+	uint32_t nf;        // Hilbert FFT size
+	uint32_t dz_up;     // dead zone, upsampled samples
+	uint32_t n1, n2;    // grid
+	uint64_t npix;
+	float offset;       // signalOffset_ (samples)
+	double inv_ct;      // (fs*U)/c, samples per metre
+	double pitch;
+	int cf_kind;
+	float cf_param;
+	int storage;
+};
+
+// Size helpers (FFTUtil.h:100-135, HilbertEnvelope.h:127).
+uint32_t next_fast_even(uint32_t n);
+inline uint32_t hilbert_fft_size(uint32_t n)
+{
+	const uint32_t a = next_fast_even(n + 1024u);
+	return a > 2048u ? a : 2048u;
+}
+// Interpolator.h:64-111 low-pass design (double precision).
+std::vector<double> upsampling_filter(uint32_t up, double half_tw);
+
+// Bytes per complex sample of a storage kind.
+inline size_t storage_bytes(int storage) { return storage == MISTA_STORAGE_F32 ? 8 : 4; }
+
+// ---- kernels launchers (defined in the .hip files) ----
+void launch_delay_table(const float* d_grid_xz, uint64_t npix, uint32_t num_elements, double pitch,
+		double inv_ct, float* d_delay, hipStream_t s);
+
+struct PrepPlan;  // hipFFT plans + scratch for signal preparation
+struct PrepState {
+	PrepPlan* plan = nullptr;  // owned; destroyed in prep.hip where PrepPlan is complete
+	PrepState() = default;
+	PrepState(const PrepState&) = delete;
+	PrepState& operator=(const PrepState&) = delete;
+	~PrepState();
+};
+void prepare_signals(PrepState& st, const FrameGeom& g, const float* d_raw, const float* d_filter_poly,
+		uint32_t taps_per_phase, uint32_t filter_offset, void* d_tensor, float* d_scale, hipStream_t s);
+
+void launch_weights(const FrameGeom& g, const float* d_rx_apod, const float* d_tx_apod, const float* d_scale,
+		float* d_w, hipStream_t s);
+
+struct DasTuning {
+	int variant = 0;  // 0 auto, 1 gather, 2 tiled
+	int p0 = 0, p1 = 0;
+};
+struct DasScratch {
+	DeviceBuffer partial;
+};
+void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, const void* d_tensor,
+		const float* d_delay, const float* d_w, float* d_value, float* d_factor, hipStream_t s);
+
+void launch_tensor_convert_in(const FrameGeom& g, const float* d_f32, void* d_tensor, float* d_scale, hipStream_t s);
+void launch_tensor_convert_out(const FrameGeom& g, const void* d_tensor, const float* d_scale, float* d_f32, hipStream_t s);
+
+}  // namespace mista

@@ -1,0 +1,471 @@
+// Signal preparation on the GPU: Kaiser-sinc upsampling, DC removal and the
+// FFT analytic signal (VectorialSTAProcessor.h:169-231, Interpolator.h:114-138,
+// Util.h:1009-1018, HilbertEnvelope.h:142-251), plus the FP64 element-to-pixel
+// delay table (replaces calculateDelaysSTAKernel, VectorialSTACUDAProcessor.cu:30-53).
+//
+// Per chunk of channels (one channel = one (tx, rx) echo):
+//   fir_dc_kernel     raw [ch][ns] -> real [ch][Nf]: polyphase FIR from LDS,
+//                     mean of [dz, L) removed, zero padding [L, Nf)
+//   hipfft R2C        real [ch][Nf] -> spec [ch][Nf/2+1]
+//   hilbert_kernel    spec -> z [ch][Nf]: DC/Nyquist x1/Nf, positive bins
+//                     x2/Nf, negative bins 0
+//   hipfft C2C (inv)  z -> z in place
+//   store_kernel      first L samples of z -> analytic tensor in the storage
+//                     type (FP16/BF16 with a per-channel scale)
+#include "common.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace mista {
+
+uint32_t next_fast_even(uint32_t n)
+{
+	if (n == 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "n must be > 0.");
+	if (n & 1) ++n;
+	for (;;) {
+		uint32_t m = n;
+		while ((m & 1) == 0) m >>= 1;
+		while (m % 3 == 0) m /= 3;
+		while (m % 5 == 0) m /= 5;
+		if (m == 1) return n;
+		n += 2;
+	}
+}
+
+// Interpolator::prepare (Interpolator.h:64-111): Kaiser tolerance 1e-4
+// (80 dB, beta = 0.1102*(80-8.7)), transition width 2*half_tw/U, size rounded
+// to k*2U+1, windowed sinc.  Evaluated in double.
+std::vector<double> upsampling_filter(uint32_t up, double half_tw)
+{
+	if (up < 2 || up > 64) MISTA_THROW(MISTA_E_INVALID_VALUE, "Invalid upsampling factor: " << up);
+	const double pi = 3.14159265358979323846;
+	const double tol_db = -20.0 * std::log10(1.0e-4);
+	const double beta = 0.1102 * (tol_db - 8.7);
+	const double tw = (half_tw * 2) / up;
+	const double wsize_d = std::ceil((tol_db - 7.95) / (2.285 * tw * pi) + 1.0);
+	const uint32_t wsize = static_cast<uint32_t>(wsize_d);
+	const double two_up = 2.0 * up;
+	const uint32_t fsize = static_cast<uint32_t>(std::ceil((wsize - 1) / two_up) * two_up) + 1U;
+	const double num_periods = (fsize - 1U) / two_up;
+	const double step = (2 * num_periods) / (fsize - 1U);
+	std::vector<double> h(fsize);
+	const double c1 = 2.0 / (fsize - 1U);
+	const double c2 = 1.0 / std::cyl_bessel_i(0.0, beta);
+	for (uint32_t i = 0; i < fsize; ++i) {
+		const double x = -num_periods + i * step;
+		const double s = (std::abs(x) < 1e-12) ? 1.0 : std::sin(pi * x) / (pi * x);
+		const double k = c1 * i - 1;
+		double a = 1 - k * k;
+		if (a < 0) a = 0;
+		h[i] = s * c2 * std::cyl_bessel_i(0.0, beta * std::sqrt(a));
+	}
+	return h;
+}
+
+// ---------------------------------------------------------------- delays
+
+// One thread per (pixel, element).  FP64 distance, rounded once to FP32
+// samples: d = sqrt((x_p - x_e)^2 + z_p^2) * (fs*U/c), x_e from
+// ArrayGeometry.h:37-49.  Layout [element][pixel] (pixel fastest).
+__global__ void __launch_bounds__(256) delay_table_kernel(const float2* __restrict__ grid, uint64_t npix,
+		uint32_t num_elements, double pitch, double inv_ct, float* __restrict__ delay)
+{
+	const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint32_t e = blockIdx.y;
+	if (p >= npix) return;
+	const float2 g = grid[p];
+	const double xe = e * pitch - ((num_elements - 1) * pitch) / 2;
+	const double dx = static_cast<double>(g.x) - xe;
+	const double z = static_cast<double>(g.y);
+	delay[static_cast<uint64_t>(e) * npix + p] = static_cast<float>(sqrt(dx * dx + z * z) * inv_ct);
+}
+
+void launch_delay_table(const float* d_grid_xz, uint64_t npix, uint32_t num_elements, double pitch,
+		double inv_ct, float* d_delay, hipStream_t s)
+{
+	if (npix == 0) return;
+	const dim3 grid(static_cast<uint32_t>((npix + 255) / 256), num_elements);
+	hipLaunchKernelGGL(delay_table_kernel, grid, dim3(256), 0, s,
+			reinterpret_cast<const float2*>(d_grid_xz), npix, num_elements, pitch, inv_ct, d_delay);
+	HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- FIR + DC
+
+constexpr int kFirThreads = 1024;
+
+// One workgroup per channel.  The raw channel sits in LDS with J zeros on the
+// left and enough on the right, the polyphase filter hp[phase][j] =
+// h[phase + U*j] (zero padded to J taps per phase) follows it.  Output
+// y[i] = sum_j hp[(i+off)%U][j] * x[(i+off)/U - j], i in [0, L); the sum over
+// [dz, L) (double) gives the mean removed from every sample; [L, Nf) is zero.
+template<int VPT>
+__global__ void __launch_bounds__(kFirThreads) fir_dc_kernel(const float* __restrict__ raw, uint32_t ns,
+		uint32_t up, uint32_t L, uint32_t nf, uint32_t dz, const float* __restrict__ hpoly, uint32_t J,
+		uint32_t off, float* __restrict__ out)
+{
+	extern __shared__ float smem[];
+	const uint32_t ch = blockIdx.x;
+	const uint32_t tid = threadIdx.x;
+	const uint32_t right = off / up + 2;
+	float* xs = smem;                               // [J + ns + right]
+	float* hs = smem + J + ns + right;              // [up][J]
+	__shared__ double red[kFirThreads / 64];
+
+	const float* src = raw + static_cast<uint64_t>(ch) * ns;
+	for (uint32_t i = tid; i < J + ns + right; i += kFirThreads) {
+		const int q = static_cast<int>(i) - static_cast<int>(J);
+		xs[i] = (q >= 0 && q < static_cast<int>(ns)) ? src[q] : 0.0f;
+	}
+	for (uint32_t i = tid; i < up * J; i += kFirThreads) hs[i] = hpoly[i];
+	__syncthreads();
+
+	float y[VPT];
+	double part = 0.0;
+#pragma unroll
+	for (int v = 0; v < VPT; ++v) {
+		const uint32_t i = tid + static_cast<uint32_t>(v) * kFirThreads;
+		float acc = 0.0f;
+		if (i < L) {
+			const uint32_t m = i + off;
+			const uint32_t ph = m % up;
+			const uint32_t q0 = m / up;
+			const float* hrow = hs + ph * J;
+			const float* xrow = xs + J + q0;
+			for (uint32_t j = 0; j < J; ++j) acc = fmaf(hrow[j], xrow[-static_cast<int>(j)], acc);
+			if (i >= dz) part += static_cast<double>(acc);
+		}
+		y[v] = acc;
+	}
+	// block reduction of the DC sum (fixed order: lanes, then waves)
+	for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
+	if ((tid & 63) == 0) red[tid >> 6] = part;
+	__syncthreads();
+	if (tid == 0) {
+		double s = 0.0;
+		for (int w = 0; w < kFirThreads / 64; ++w) s += red[w];
+		red[0] = s;
+	}
+	__syncthreads();
+	const float mean = static_cast<float>(red[0] / static_cast<double>(L - dz));
+
+	float* dst = out + static_cast<uint64_t>(ch) * nf;
+#pragma unroll
+	for (int v = 0; v < VPT; ++v) {
+		const uint32_t i = tid + static_cast<uint32_t>(v) * kFirThreads;
+		if (i < L) dst[i] = y[v] - mean;
+	}
+	for (uint32_t i = L + tid; i < nf; i += kFirThreads) dst[i] = 0.0f;
+}
+
+// U = 1: copy + DC removal (Util::removeDC only).
+__global__ void __launch_bounds__(kFirThreads) copy_dc_kernel(const float* __restrict__ raw, uint32_t L,
+		uint32_t nf, uint32_t dz, float* __restrict__ out)
+{
+	__shared__ double red[kFirThreads / 64];
+	const uint32_t ch = blockIdx.x;
+	const uint32_t tid = threadIdx.x;
+	const float* src = raw + static_cast<uint64_t>(ch) * L;
+	double part = 0.0;
+	for (uint32_t i = dz + tid; i < L; i += kFirThreads) part += static_cast<double>(src[i]);
+	for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
+	if ((tid & 63) == 0) red[tid >> 6] = part;
+	__syncthreads();
+	if (tid == 0) {
+		double s = 0.0;
+		for (int w = 0; w < kFirThreads / 64; ++w) s += red[w];
+		red[0] = s;
+	}
+	__syncthreads();
+	const float mean = static_cast<float>(red[0] / static_cast<double>(L - dz));
+	float* dst = out + static_cast<uint64_t>(ch) * nf;
+	for (uint32_t i = tid; i < nf; i += kFirThreads) dst[i] = (i < L) ? src[i] - mean : 0.0f;
+}
+
+// ---------------------------------------------------------------- Hilbert
+
+// spec [ch][Nf/2+1] -> z [ch][Nf]: bin 0 and Nf/2 scaled by 1/Nf (the
+// reference's 0.5 then 2/Nf), bins 1..Nf/2-1 by 2/Nf, bins > Nf/2 zeroed.
+__global__ void __launch_bounds__(256) hilbert_kernel(const float2* __restrict__ spec, uint32_t nf,
+		float2* __restrict__ z)
+{
+	const uint32_t ch = blockIdx.y;
+	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= nf) return;
+	const uint32_t half = nf / 2;
+	const float inv = 1.0f / static_cast<float>(nf);
+	float2 v = make_float2(0.0f, 0.0f);
+	if (k <= half) {
+		const float2 s = spec[static_cast<uint64_t>(ch) * (half + 1) + k];
+		const float c = (k == 0 || k == half) ? inv : 2.0f * inv;
+		v = make_float2(s.x * c, s.y * c);
+	}
+	z[static_cast<uint64_t>(ch) * nf + k] = v;
+}
+
+// ---------------------------------------------------------------- storage
+
+__device__ inline uint32_t pack_bf16x2(float re, float im)
+{
+	// round to nearest even
+	auto rne = [](float f) -> uint32_t {
+		uint32_t b = __float_as_uint(f);
+		b += 0x7FFFu + ((b >> 16) & 1u);
+		return b >> 16;
+	};
+	return rne(re) | (rne(im) << 16);
+}
+
+// One workgroup per channel.  F32: copy.  F16/BF16: per-channel scale
+// s = max(|re|, |im|) over the channel (1 if zero), stored value z/s.
+template<int STORAGE>
+__global__ void __launch_bounds__(256) store_kernel(const float2* __restrict__ z, uint32_t nf, uint32_t L,
+		void* __restrict__ tensor, float* __restrict__ scale, uint64_t ch0)
+{
+	const uint32_t ch = blockIdx.x;
+	const uint32_t tid = threadIdx.x;
+	const float2* src = z + static_cast<uint64_t>(ch) * nf;
+	const uint64_t gch = ch0 + ch;
+	if constexpr (STORAGE == MISTA_STORAGE_F32) {
+		float2* dst = static_cast<float2*>(tensor) + gch * L;
+		for (uint32_t i = tid; i < L; i += 256) dst[i] = src[i];
+	} else {
+		__shared__ float red[4];
+		float m = 0.0f;
+		for (uint32_t i = tid; i < L; i += 256) {
+			const float2 v = src[i];
+			m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+		}
+		for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_down(m, o, 64));
+		if ((tid & 63) == 0) red[tid >> 6] = m;
+		__syncthreads();
+		const float mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+		const float s = (mx > 0.0f) ? mx : 1.0f;
+		const float inv = 1.0f / s;
+		if (tid == 0) scale[gch] = s;
+		if constexpr (STORAGE == MISTA_STORAGE_F16) {
+			__half2* dst = static_cast<__half2*>(tensor) + gch * L;
+			for (uint32_t i = tid; i < L; i += 256) {
+				const float2 v = src[i];
+				dst[i] = __floats2half2_rn(v.x * inv, v.y * inv);
+			}
+		} else {
+			uint32_t* dst = static_cast<uint32_t*>(tensor) + gch * L;
+			for (uint32_t i = tid; i < L; i += 256) {
+				const float2 v = src[i];
+				dst[i] = pack_bf16x2(v.x * inv, v.y * inv);
+			}
+		}
+	}
+}
+
+// Test conversion back to float32 (per-channel scale applied).
+template<int STORAGE>
+__global__ void __launch_bounds__(256) convert_out_kernel(const void* __restrict__ tensor, const float* __restrict__ scale,
+		uint32_t L, float2* __restrict__ dst)
+{
+	const uint32_t ch = blockIdx.x;
+	const uint32_t tid = threadIdx.x;
+	const uint64_t base = static_cast<uint64_t>(ch) * L;
+	if constexpr (STORAGE == MISTA_STORAGE_F32) {
+		const float2* s = static_cast<const float2*>(tensor) + base;
+		for (uint32_t i = tid; i < L; i += 256) dst[base + i] = s[i];
+	} else if constexpr (STORAGE == MISTA_STORAGE_F16) {
+		const __half2* s = static_cast<const __half2*>(tensor) + base;
+		const float sc = scale[ch];
+		for (uint32_t i = tid; i < L; i += 256) {
+			const float2 v = __half22float2(s[i]);
+			dst[base + i] = make_float2(v.x * sc, v.y * sc);
+		}
+	} else {
+		const uint32_t* s = static_cast<const uint32_t*>(tensor) + base;
+		const float sc = scale[ch];
+		for (uint32_t i = tid; i < L; i += 256) {
+			const uint32_t b = s[i];
+			dst[base + i] = make_float2(__uint_as_float(b << 16) * sc, __uint_as_float(b & 0xFFFF0000u) * sc);
+		}
+	}
+}
+
+void launch_tensor_convert_in(const FrameGeom& g, const float* d_f32, void* d_tensor, float* d_scale, hipStream_t s)
+{
+	const uint32_t nch = g.ntx * g.nrx;
+	const float2* src = reinterpret_cast<const float2*>(d_f32);
+	switch (g.storage) {
+	case MISTA_STORAGE_F32:
+		hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_F32>, dim3(nch), dim3(256), 0, s, src, g.L, g.L, d_tensor, d_scale, 0ull);
+		break;
+	case MISTA_STORAGE_F16:
+		hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_F16>, dim3(nch), dim3(256), 0, s, src, g.L, g.L, d_tensor, d_scale, 0ull);
+		break;
+	default:
+		hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_BF16>, dim3(nch), dim3(256), 0, s, src, g.L, g.L, d_tensor, d_scale, 0ull);
+		break;
+	}
+	HIP_CHECK(hipGetLastError());
+}
+
+void launch_tensor_convert_out(const FrameGeom& g, const void* d_tensor, const float* d_scale, float* d_f32, hipStream_t s)
+{
+	const uint32_t nch = g.ntx * g.nrx;
+	float2* dst = reinterpret_cast<float2*>(d_f32);
+	switch (g.storage) {
+	case MISTA_STORAGE_F32:
+		hipLaunchKernelGGL(convert_out_kernel<MISTA_STORAGE_F32>, dim3(nch), dim3(256), 0, s, d_tensor, d_scale, g.L, dst);
+		break;
+	case MISTA_STORAGE_F16:
+		hipLaunchKernelGGL(convert_out_kernel<MISTA_STORAGE_F16>, dim3(nch), dim3(256), 0, s, d_tensor, d_scale, g.L, dst);
+		break;
+	default:
+		hipLaunchKernelGGL(convert_out_kernel<MISTA_STORAGE_BF16>, dim3(nch), dim3(256), 0, s, d_tensor, d_scale, g.L, dst);
+		break;
+	}
+	HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- weights
+
+// w[t][r] = rxApod[r] * txApod[t] * scale[t][r]; txApod/scale optional.
+__global__ void __launch_bounds__(256) weights_kernel(uint32_t ntx, uint32_t nrx, const float* __restrict__ rx_apod,
+		const float* __restrict__ tx_apod, const float* __restrict__ scale, float* __restrict__ w)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= ntx * nrx) return;
+	const uint32_t t = i / nrx, r = i % nrx;
+	float v = rx_apod[r];
+	if (tx_apod) v = v * tx_apod[t];
+	if (scale) v = v * scale[i];
+	w[i] = v;
+}
+
+void launch_weights(const FrameGeom& g, const float* d_rx_apod, const float* d_tx_apod, const float* d_scale,
+		float* d_w, hipStream_t s)
+{
+	const uint32_t n = g.ntx * g.nrx;
+	hipLaunchKernelGGL(weights_kernel, dim3((n + 255) / 256), dim3(256), 0, s, g.ntx, g.nrx, d_rx_apod, d_tx_apod,
+			g.storage == MISTA_STORAGE_F32 ? nullptr : d_scale, d_w);
+	HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- driver
+
+struct PrepPlan {
+	uint32_t nf = 0, chunk = 0, tail = 0;
+	hipfftHandle r2c = 0, c2c = 0, r2c_tail = 0, c2c_tail = 0;
+	bool has_tail = false;
+	DeviceBuffer real, spec, z;
+	~PrepPlan()
+	{
+		if (r2c) hipfftDestroy(r2c);
+		if (c2c) hipfftDestroy(c2c);
+		if (r2c_tail) hipfftDestroy(r2c_tail);
+		if (c2c_tail) hipfftDestroy(c2c_tail);
+	}
+};
+
+PrepState::~PrepState() { delete plan; }
+
+static void make_plans(hipfftHandle* r2c, hipfftHandle* c2c, uint32_t nf, uint32_t batch, hipStream_t s)
+{
+	int n[1] = {static_cast<int>(nf)};
+	FFT_CHECK(hipfftPlanMany(r2c, 1, n, nullptr, 1, static_cast<int>(nf), nullptr, 1,
+			static_cast<int>(nf / 2 + 1), HIPFFT_R2C, static_cast<int>(batch)));
+	FFT_CHECK(hipfftPlanMany(c2c, 1, n, nullptr, 1, static_cast<int>(nf), nullptr, 1, static_cast<int>(nf),
+			HIPFFT_C2C, static_cast<int>(batch)));
+	FFT_CHECK(hipfftSetStream(*r2c, s));
+	FFT_CHECK(hipfftSetStream(*c2c, s));
+}
+
+template<int VPT>
+static void launch_fir(uint32_t nch, const FrameGeom& g, const float* raw, const float* hpoly, uint32_t J,
+		uint32_t off, float* out, hipStream_t s)
+{
+	const uint32_t right = off / g.up + 2;
+	const size_t lds = (static_cast<size_t>(J) + g.ns + right + static_cast<size_t>(g.up) * J) * sizeof(float);
+	HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(fir_dc_kernel<VPT>),
+			hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+	hipLaunchKernelGGL(fir_dc_kernel<VPT>, dim3(nch), dim3(kFirThreads), lds, s, raw, g.ns, g.up, g.L, g.nf, g.dz_up,
+			hpoly, J, off, out);
+}
+
+void prepare_signals(PrepState& st, const FrameGeom& g, const float* d_raw, const float* d_filter_poly,
+		uint32_t J, uint32_t off, void* d_tensor, float* d_scale, hipStream_t s)
+{
+	const uint32_t nch = g.ntx * g.nrx;
+	const uint32_t chunk = std::min<uint32_t>(nch, 1024u);
+	if (!st.plan || st.plan->nf != g.nf || st.plan->chunk != chunk || st.plan->tail != nch % chunk) {
+		delete st.plan;
+		st.plan = nullptr;
+		auto p = std::make_unique<PrepPlan>();
+		p->nf = g.nf;
+		p->chunk = chunk;
+		p->tail = nch % chunk;
+		make_plans(&p->r2c, &p->c2c, g.nf, chunk, s);
+		if (p->tail) {
+			make_plans(&p->r2c_tail, &p->c2c_tail, g.nf, p->tail, s);
+			p->has_tail = true;
+		}
+		st.plan = p.release();
+	}
+	PrepPlan& P = *st.plan;
+	FFT_CHECK(hipfftSetStream(P.r2c, s));
+	FFT_CHECK(hipfftSetStream(P.c2c, s));
+	if (P.has_tail) {
+		FFT_CHECK(hipfftSetStream(P.r2c_tail, s));
+		FFT_CHECK(hipfftSetStream(P.c2c_tail, s));
+	}
+	float* real = static_cast<float*>(P.real.reserve(static_cast<size_t>(chunk) * g.nf * sizeof(float)));
+	float2* spec = static_cast<float2*>(P.spec.reserve(static_cast<size_t>(chunk) * (g.nf / 2 + 1) * sizeof(float2)));
+	float2* z = static_cast<float2*>(P.z.reserve(static_cast<size_t>(chunk) * g.nf * sizeof(float2)));
+
+	if (g.up > 1) {
+		const size_t lds = (static_cast<size_t>(J) + g.ns + off / g.up + 2 + static_cast<size_t>(g.up) * J) * sizeof(float);
+		if (lds > 150 * 1024) MISTA_THROW(MISTA_E_INVALID_PARAM, "channel too long for the LDS FIR: " << g.ns << " samples");
+	}
+	const uint32_t vpt = (g.L + kFirThreads - 1) / kFirThreads;
+	if (vpt > 64) MISTA_THROW(MISTA_E_INVALID_PARAM, "signal length " << g.L << " exceeds 65536");
+
+	for (uint32_t c0 = 0; c0 < nch; c0 += chunk) {
+		const uint32_t n = std::min(chunk, nch - c0);
+		const float* raw = d_raw + static_cast<uint64_t>(c0) * g.ns;
+		if (g.up == 1) {
+			hipLaunchKernelGGL(copy_dc_kernel, dim3(n), dim3(kFirThreads), 0, s, raw, g.L, g.nf, g.dz_up, real);
+		} else if (vpt <= 4) {
+			launch_fir<4>(n, g, raw, d_filter_poly, J, off, real, s);
+		} else if (vpt <= 8) {
+			launch_fir<8>(n, g, raw, d_filter_poly, J, off, real, s);
+		} else if (vpt <= 16) {
+			launch_fir<16>(n, g, raw, d_filter_poly, J, off, real, s);
+		} else if (vpt <= 32) {
+			launch_fir<32>(n, g, raw, d_filter_poly, J, off, real, s);
+		} else {
+			launch_fir<64>(n, g, raw, d_filter_poly, J, off, real, s);
+		}
+		HIP_CHECK(hipGetLastError());
+		hipfftHandle r2c = (n == chunk) ? P.r2c : P.r2c_tail;
+		hipfftHandle c2c = (n == chunk) ? P.c2c : P.c2c_tail;
+		FFT_CHECK(hipfftExecR2C(r2c, real, reinterpret_cast<hipfftComplex*>(spec)));
+		hipLaunchKernelGGL(hilbert_kernel, dim3((g.nf + 255) / 256, n), dim3(256), 0, s, spec, g.nf, z);
+		HIP_CHECK(hipGetLastError());
+		FFT_CHECK(hipfftExecC2C(c2c, reinterpret_cast<hipfftComplex*>(z), reinterpret_cast<hipfftComplex*>(z),
+				HIPFFT_BACKWARD));
+		switch (g.storage) {
+		case MISTA_STORAGE_F32:
+			hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_F32>, dim3(n), dim3(256), 0, s, z, g.nf, g.L, d_tensor, d_scale,
+					static_cast<uint64_t>(c0));
+			break;
+		case MISTA_STORAGE_F16:
+			hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_F16>, dim3(n), dim3(256), 0, s, z, g.nf, g.L, d_tensor, d_scale,
+					static_cast<uint64_t>(c0));
+			break;
+		default:
+			hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_BF16>, dim3(n), dim3(256), 0, s, z, g.nf, g.L, d_tensor, d_scale,
+					static_cast<uint64_t>(c0));
+			break;
+		}
+		HIP_CHECK(hipGetLastError());
+	}
+}
+
+}  // namespace mista

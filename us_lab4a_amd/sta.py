@@ -1,0 +1,230 @@
+"""STA processor with the reference's interface, running on the mista HIP library.
+
+Mirrors VectorialSTAProcessor / VectorialSTACUDAProcessor
+(src/user/processor/VectorialSTAProcessor.h:58-341,
+VectorialSTACUDAProcessor.h:40-127): constructor(config, acquisition,
+upsampling_factor, coherence_factor, peak_offset, rx_apod), prepare(base) and
+process(grid_data) writing value/factor into a Matrix<XYZValueFactor<float>>
+equivalent (numpy structured array, imaging.XYZ_VALUE_FACTOR).
+
+All arithmetic runs in libmista_hip.so; this module only moves arguments.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .imaging import grid_xz
+
+_STORAGE = {"f32": _lib.STORAGE_F32, "f16": _lib.STORAGE_F16, "bf16": _lib.STORAGE_BF16}
+
+
+@dataclass
+class STAConfiguration:
+    """Fields of STAConfiguration<float> on the path (STAConfiguration.h:118-165)."""
+    num_elements: int
+    pitch: float
+    sampling_frequency: float
+    propagation_speed: float
+    center_frequency: float
+    first_tx_elem: int = 0
+    last_tx_elem: int = -1          # -1: num_elements - 1
+    dead_zone_m: float = 0.0
+    max_frequency: float = 0.0
+
+    def __post_init__(self):
+        if self.last_tx_elem < 0:
+            self.last_tx_elem = self.num_elements - 1
+
+    @property
+    def num_tx(self):
+        return self.last_tx_elem - self.first_tx_elem + 1
+
+
+@dataclass
+class CoherenceFactor:
+    """AnalyticSignalCoherenceFactorProcessor selection (CoherenceFactor.h:139-152):
+    method "none" | "phase_coherence_factor" (param = gamma) |
+    "sign_coherence_factor" (param = p, applied to Re of the per-rx sums)."""
+    method: str = "none"
+    param: float = 1.0
+
+    @property
+    def kind(self):
+        return {"none": _lib.CF_NONE, "phase_coherence_factor": _lib.CF_PCF,
+                "sign_coherence_factor": _lib.CF_SCF}[self.method]
+
+    def enabled(self):
+        return self.method != "none"
+
+
+def _ptr(a):
+    """Pointer of a numpy array or a torch tensor (host or device), or an int."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return stream.cuda_stream
+    return int(stream)
+
+
+class HipSTAProcessor:
+    """ArrayProcessor<XYZValueFactor<float>> on MI355X (ArrayProcessor.h:27-40)."""
+
+    def __init__(self, config, acquisition, upsampling_factor, coherence_factor, peak_offset, rx_apod,
+                 tx_apod=None, storage="f32", device=0):
+        self.config = config
+        self.acquisition = acquisition
+        self.coherence_factor = coherence_factor if coherence_factor is not None else CoherenceFactor()
+        c = _lib.StaConfig()
+        c.num_elements = config.num_elements
+        c.first_tx_elem = config.first_tx_elem
+        c.last_tx_elem = config.last_tx_elem
+        c.upsampling_factor = upsampling_factor
+        c.pitch = config.pitch
+        c.sampling_frequency = config.sampling_frequency
+        c.propagation_speed = config.propagation_speed
+        c.center_frequency = config.center_frequency
+        c.dead_zone_m = config.dead_zone_m
+        c.peak_offset = peak_offset
+        c.cf_kind = self.coherence_factor.kind
+        c.cf_param = self.coherence_factor.param
+        c.storage = _STORAGE[storage]
+        c.device = device
+        self._cfg = c
+        h = ctypes.c_void_p()
+        check(lib().mista_create(ctypes.byref(c), ctypes.byref(h)))
+        self._h = h
+        self.upsampling_factor = upsampling_factor
+        self.storage = storage
+        self._grid_shape = None
+        self.set_rx_apod(rx_apod)
+        if tx_apod is not None:
+            self.set_tx_apod(tx_apod)
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mista_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- configuration ----------------------------------------------------
+    def set_rx_apod(self, rx_apod):
+        a = np.ascontiguousarray(rx_apod, dtype=np.float32)
+        check(lib().mista_set_rx_apod(self._h, a.ctypes.data, a.size), self._h)
+
+    def set_tx_apod(self, tx_apod):
+        if tx_apod is None:
+            check(lib().mista_set_tx_apod(self._h, None, 0), self._h)
+            return
+        a = np.ascontiguousarray(tx_apod, dtype=np.float32)
+        check(lib().mista_set_tx_apod(self._h, a.ctypes.data, a.size), self._h)
+
+    def set_grid(self, grid):
+        """grid: structured XYZValueFactor array [n1][n2] or float32 [n1][n2][2] (x, z)."""
+        xz = grid_xz(grid) if grid.dtype.names else np.ascontiguousarray(grid, dtype=np.float32)
+        n1, n2 = xz.shape[:2]
+        check(lib().mista_set_grid(self._h, xz.ctypes.data, n1, n2), self._h)
+        self._grid_shape = (n1, n2)
+        self._grid_xz = xz
+
+    def set_kernel(self, variant, p0=0, p1=0):
+        check(lib().mista_set_kernel(self._h, variant, p0, p1), self._h)
+
+    # -- reference interface ----------------------------------------------
+    def prepare(self, base_element):
+        """ArrayProcessor::prepare (VectorialSTAProcessor.h:151-156)."""
+        self.acquisition.prepare(base_element)
+        check(lib().mista_prepare(self._h, base_element), self._h)
+
+    def process(self, grid_data):
+        """VectorialSTAProcessor::process: pulls every tx from the acquisition,
+        forms the image and writes value/factor into grid_data in place."""
+        xz = grid_xz(grid_data)
+        if self._grid_shape != xz.shape[:2] or not np.array_equal(xz, self._grid_xz):
+            self.set_grid(grid_data)
+        cfg = self.config
+        first = self.acquisition.execute(cfg.first_tx_elem)
+        echoes = np.empty((cfg.num_tx,) + first.shape, dtype=np.float32)
+        echoes[0] = first
+        for t in range(1, cfg.num_tx):
+            echoes[t] = self.acquisition.execute(cfg.first_tx_elem + t)
+        value, factor = self.process_echoes(echoes)
+        grid_data["value"] = value
+        grid_data["factor"] = factor
+
+    def process_echoes(self, echoes):
+        """Host frame [ntx][nrx][ns] float32 -> (value, factor) [n1][n2]."""
+        echoes = np.ascontiguousarray(echoes, dtype=np.float32)
+        if self._grid_shape is None:
+            raise _lib.InvalidStateError(_lib.MISTA_E_INVALID_STATE, "The image grid has not been set.")
+        n1, n2 = self._grid_shape
+        value = np.empty((n1, n2), np.float32)
+        factor = np.empty((n1, n2), np.float32)
+        check(lib().mista_process(self._h, echoes.ctypes.data, echoes.shape[-1], value.ctypes.data,
+                                  factor.ctypes.data), self._h)
+        return value, factor
+
+    # -- device-resident path ---------------------------------------------
+    def process_device(self, d_echoes, num_samples, d_value, d_factor=None, stream=None):
+        check(lib().mista_process_device(self._h, _ptr(d_echoes), num_samples, _ptr(d_value), _ptr(d_factor),
+                                         _stream_ptr(stream)), self._h)
+
+    def prepare_signals_device(self, d_echoes, num_samples, stream=None):
+        check(lib().mista_prepare_signals_device(self._h, _ptr(d_echoes), num_samples, _stream_ptr(stream)), self._h)
+
+    def das_device(self, d_value, d_factor=None, stream=None):
+        check(lib().mista_das_device(self._h, _ptr(d_value), _ptr(d_factor), _stream_ptr(stream)), self._h)
+
+    def upload_analytic(self, analytic):
+        """analytic: complex [ntx][nrx][L] (complex64/128) or float32 [..][L][2]."""
+        a = np.asarray(analytic)
+        if np.iscomplexobj(a):
+            a = np.stack([a.real, a.imag], axis=-1)
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        check(lib().mista_upload_analytic(self._h, a.ctypes.data, a.shape[2]), self._h)
+
+    def download_analytic(self):
+        L = self.query()[0]
+        out = np.empty((self.config.num_tx, self.config.num_elements, L, 2), np.float32)
+        check(lib().mista_download_analytic(self._h, out.ctypes.data), self._h)
+        return out
+
+    def download_delays(self):
+        n1, n2 = self._grid_shape
+        out = np.empty((self.config.num_elements, n1 * n2), np.float32)
+        check(lib().mista_download_delays(self._h, out.ctypes.data), self._h)
+        return out
+
+    def query(self):
+        L, nf, npts = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        check(lib().mista_query(self._h, ctypes.byref(L), ctypes.byref(nf), ctypes.byref(npts)), self._h)
+        return L.value, nf.value, npts.value
+
+    def last_timing(self):
+        p, d = ctypes.c_float(), ctypes.c_float()
+        check(lib().mista_last_timing(self._h, ctypes.byref(p), ctypes.byref(d)), self._h)
+        return p.value, d.value
