@@ -237,6 +237,30 @@ def test_graded_shape_columns_match_oracle():
     p.close()
 
 
+@pytest.mark.parametrize("tz,storage", [(0, "f32"), (2, "f32"), (3, "f32"), (0, "f16"), (0, "bf16")])
+def test_tiled_kernel_equals_gather_kernel(small, tz, storage):
+    """The LDS-tiled kernel uses the gather kernel's arithmetic and, with a
+    coherence factor, its summation order: results are bit-identical."""
+    import torch
+    wl, echoes = small
+    wl.cf = "phase_coherence_factor"
+    say("tiled vs gather, tile", tz, storage)
+    p = make_proc(wl, storage=storage)
+    p.process_echoes(echoes)
+    n = wl.n_x * wl.n_z
+    out = {}
+    for variant, p0 in ((1, 0), (2, tz)):
+        p.set_kernel(variant, p0, 4)
+        dv = torch.empty(n, dtype=torch.float32, device="cuda")
+        df = torch.empty(n, dtype=torch.float32, device="cuda")
+        p.das_device(dv, df)
+        torch.cuda.synchronize()
+        out[variant] = (dv.cpu().numpy(), df.cpu().numpy())
+    assert np.array_equal(out[1][0], out[2][0]) and np.array_equal(out[1][1], out[2][1])
+    p.close()
+    wl.cf = "none"
+
+
 def test_error_paths_raise_reference_style_errors():
     wl = workloads.tiny()
     say("error paths")

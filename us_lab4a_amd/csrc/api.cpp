@@ -34,6 +34,7 @@ struct mista_handle {
 
 	// grid + resident delay table
 	uint32_t n1 = 0, n2 = 0;
+	uint64_t grid_version = 0;
 	DeviceBuffer d_grid, d_delay;
 
 	// frame state
@@ -119,13 +120,14 @@ uint32_t dead_zone_up(const mista_sta_config& c)
 
 // Shapes of a frame with signal length L (ns raw samples; ns == 0 when the
 // tensor is uploaded directly).
-void set_frame_geom(mista_handle* h, uint32_t ns, uint32_t L)
+void set_frame_geom(mista_handle* h, uint32_t ns, uint32_t L, hipStream_t s)
 {
 	const mista_sta_config& c = h->cfg;
 	FrameGeom& g = h->geom;
 	g.num_elements = c.num_elements;
 	g.first_tx = c.first_tx_elem;
 	g.ntx = ntx_of(c);
+	g.ntx_pad = pad_ntx(g.ntx);
 	g.nrx = c.num_elements;
 	g.ns = ns;
 	g.up = c.upsampling_factor;
@@ -138,6 +140,7 @@ void set_frame_geom(mista_handle* h, uint32_t ns, uint32_t L)
 	g.n1 = h->n1;
 	g.n2 = h->n2;
 	g.npix = static_cast<uint64_t>(h->n1) * h->n2;
+	g.grid_version = h->grid_version;
 	// VectorialSTAProcessor.h:147 (float): (fs*U) * peakOffset / fc
 	g.offset = (c.sampling_frequency * static_cast<float>(c.upsampling_factor)) * c.peak_offset / c.center_frequency;
 	g.inv_ct = (static_cast<double>(c.sampling_frequency) * c.upsampling_factor) / static_cast<double>(c.propagation_speed);
@@ -146,9 +149,12 @@ void set_frame_geom(mista_handle* h, uint32_t ns, uint32_t L)
 	g.cf_param = c.cf_param;
 	g.storage = c.storage;
 	const size_t nch = static_cast<size_t>(g.ntx) * g.nrx;
-	h->d_tensor.reserve(nch * L * storage_bytes(c.storage));
+	const size_t tbytes = nch * L * storage_bytes(c.storage);
+	unsigned char* t = static_cast<unsigned char*>(h->d_tensor.reserve(tbytes + tensor_tail_padding()));
+	// the staged-window copies may read into the padding: keep it zero
+	HIP_CHECK(hipMemsetAsync(t + tbytes, 0, tensor_tail_padding(), s));
 	h->d_scale.reserve(nch * sizeof(float));
-	h->d_w.reserve(nch * sizeof(float));
+	h->d_w.reserve(static_cast<size_t>(g.ntx_pad) * g.nrx * sizeof(float));
 }
 
 void ensure_filter(mista_handle* h, hipStream_t s)
@@ -203,7 +209,7 @@ void run_prepare(mista_handle* h, const float* d_echoes, uint32_t ns, hipStream_
 	if (ns == 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "The number of input samples is 0.");
 	const uint64_t L64 = static_cast<uint64_t>(ns) * h->cfg.upsampling_factor;
 	if (L64 > (1u << 20)) MISTA_THROW(MISTA_E_INVALID_PARAM, "The number of input samples (" << L64 << ") is greater than 1048576.");
-	set_frame_geom(h, ns, static_cast<uint32_t>(L64));
+	set_frame_geom(h, ns, static_cast<uint32_t>(L64), s);
 	ensure_filter(h, s);
 	ensure_events(h);
 	HIP_CHECK(hipEventRecord(h->ev[0], s));
@@ -217,11 +223,11 @@ void run_das(mista_handle* h, float* d_value, float* d_factor, hipStream_t s)
 {
 	require_grid(h);
 	if (!h->signal_ready) MISTA_THROW(MISTA_E_INVALID_STATE, "No prepared signals: call a prepare/process/upload entry point first.");
-	if (h->geom.npix != static_cast<uint64_t>(h->n1) * h->n2) {
-		h->geom.n1 = h->n1;
-		h->geom.n2 = h->n2;
-		h->geom.npix = static_cast<uint64_t>(h->n1) * h->n2;
-	}
+	// the grid may have changed since the signals were prepared
+	h->geom.n1 = h->n1;
+	h->geom.n2 = h->n2;
+	h->geom.npix = static_cast<uint64_t>(h->n1) * h->n2;
+	h->geom.grid_version = h->grid_version;
 	upload_apod(h, s);
 	ensure_events(h);
 	// w[tx][rx] = rxApod * txApod * per-channel storage scale
@@ -318,6 +324,9 @@ int mista_set_grid(mista_handle* h, const float* grid_xz, uint32_t n1, uint32_t 
 	return guarded(h, [&] {
 		if (!grid_xz || n1 == 0 || n2 == 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "Empty image grid.");
 		const uint64_t npix = static_cast<uint64_t>(n1) * n2;
+		for (uint64_t i = 0; i < 2 * npix; ++i)
+			if (!std::isfinite(grid_xz[i]) || std::fabs(grid_xz[i]) > 1.0e3f)
+				MISTA_THROW(MISTA_E_INVALID_VALUE, "Grid coordinate " << i / 2 << " is not finite or exceeds 1 km.");
 		hipStream_t s = h->stream;
 		void* d_grid = h->d_grid.reserve(npix * 2 * sizeof(float));
 		HIP_CHECK(hipMemcpyAsync(d_grid, grid_xz, npix * 2 * sizeof(float), hipMemcpyHostToDevice, s));
@@ -329,6 +338,7 @@ int mista_set_grid(mista_handle* h, const float* grid_xz, uint32_t n1, uint32_t 
 		HIP_CHECK(hipStreamSynchronize(s));
 		h->n1 = n1;
 		h->n2 = n2;
+		++h->grid_version;
 	});
 }
 
@@ -418,7 +428,7 @@ int mista_upload_analytic(mista_handle* h, const float* analytic_ri, uint32_t si
 		if (!analytic_ri) MISTA_THROW(MISTA_E_INVALID_PARAM, "null host pointer.");
 		require_grid(h);
 		hipStream_t s = h->stream;
-		set_frame_geom(h, 0, signal_length);
+		set_frame_geom(h, 0, signal_length, s);
 		const size_t nch = static_cast<size_t>(h->geom.ntx) * h->geom.nrx;
 		const size_t bytes = nch * signal_length * 2 * sizeof(float);
 		float* d_tmp = static_cast<float*>(h->d_tmp.reserve(bytes));

@@ -72,6 +72,7 @@ struct FrameGeom {
 	uint32_t num_elements;
 	uint32_t first_tx;
 	uint32_t ntx;
+	uint32_t ntx_pad;   // transmits padded to 32/64/128 (multiple of 8 above)
 	uint32_t nrx;
 	uint32_t ns;        // raw samples per channel
 	uint32_t up;        // upsampling factor
@@ -86,6 +87,7 @@ struct FrameGeom {
 	int cf_kind;
 	float cf_param;
 	int storage;
+	uint64_t grid_version;  // bumped whenever the grid / delay table changes
 };
 
 // Size helpers (FFTUtil.h:100-135, HilbertEnvelope.h:127).
@@ -97,6 +99,15 @@ inline uint32_t hilbert_fft_size(uint32_t n)
 }
 // Interpolator.h:64-111 low-pass design (double precision).
 std::vector<double> upsampling_filter(uint32_t up, double half_tw);
+
+// Transmit count padded to the DAS kernel's instantiated sizes.
+inline uint32_t pad_ntx(uint32_t ntx)
+{
+	if (ntx <= 32) return 32;
+	if (ntx <= 64) return 64;
+	if (ntx <= 128) return 128;
+	return (ntx + 7) / 8 * 8;
+}
 
 // Bytes per complex sample of a storage kind.
 inline size_t storage_bytes(int storage) { return storage == MISTA_STORAGE_F32 ? 8 : 4; }
@@ -124,8 +135,17 @@ struct DasTuning {
 	int p0 = 0, p1 = 0;
 };
 struct DasScratch {
-	DeviceBuffer partial;
+	DeviceBuffer partial;           // small device scalars (max window slots)
+	DeviceBuffer bounds;            // per-tile element delay bounds
+	DeviceBuffer params;            // per (tile, rx, tx) window parameters
+	uint64_t bounds_version = ~0ull;
+	uint64_t params_key = 0;
+	float params_offset = 0.0f;
+	uint32_t wcap = 0;              // LDS window entries per staged pair
+	uint32_t max_slots = 0;
 };
+// Bytes of zeroed padding after the analytic tensor (window over-read).
+size_t tensor_tail_padding();
 void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, const void* d_tensor,
 		const float* d_delay, const float* d_w, float* d_value, float* d_factor, hipStream_t s);
 

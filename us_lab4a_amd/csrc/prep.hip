@@ -326,25 +326,31 @@ void launch_tensor_convert_out(const FrameGeom& g, const void* d_tensor, const f
 
 // ---------------------------------------------------------------- weights
 
-// w[t][r] = rxApod[r] * txApod[t] * scale[t][r]; txApod/scale optional.
-__global__ void __launch_bounds__(256) weights_kernel(uint32_t ntx, uint32_t nrx, const float* __restrict__ rx_apod,
-		const float* __restrict__ tx_apod, const float* __restrict__ scale, float* __restrict__ w)
+// w[r][t] = rxApod[r] * txApod[t] * scale[t][r], rx-major with the tx row
+// padded to ntx_pad zeros (the tx loop reads consecutive weights);
+// txApod/scale optional.
+__global__ void __launch_bounds__(256) weights_kernel(uint32_t ntx, uint32_t ntx_pad, uint32_t nrx,
+		const float* __restrict__ rx_apod, const float* __restrict__ tx_apod, const float* __restrict__ scale,
+		float* __restrict__ w)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= ntx * nrx) return;
-	const uint32_t t = i / nrx, r = i % nrx;
-	float v = rx_apod[r];
-	if (tx_apod) v = v * tx_apod[t];
-	if (scale) v = v * scale[i];
+	if (i >= ntx_pad * nrx) return;
+	const uint32_t r = i / ntx_pad, t = i % ntx_pad;
+	float v = 0.0f;
+	if (t < ntx) {
+		v = rx_apod[r];
+		if (tx_apod) v = v * tx_apod[t];
+		if (scale) v = v * scale[static_cast<uint64_t>(t) * nrx + r];
+	}
 	w[i] = v;
 }
 
 void launch_weights(const FrameGeom& g, const float* d_rx_apod, const float* d_tx_apod, const float* d_scale,
 		float* d_w, hipStream_t s)
 {
-	const uint32_t n = g.ntx * g.nrx;
-	hipLaunchKernelGGL(weights_kernel, dim3((n + 255) / 256), dim3(256), 0, s, g.ntx, g.nrx, d_rx_apod, d_tx_apod,
-			g.storage == MISTA_STORAGE_F32 ? nullptr : d_scale, d_w);
+	const uint32_t n = g.ntx_pad * g.nrx;
+	hipLaunchKernelGGL(weights_kernel, dim3((n + 255) / 256), dim3(256), 0, s, g.ntx, g.ntx_pad, g.nrx, d_rx_apod,
+			d_tx_apod, g.storage == MISTA_STORAGE_F32 ? nullptr : d_scale, d_w);
 	HIP_CHECK(hipGetLastError());
 }
 
