@@ -547,7 +547,7 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 			scratch.params_key = key;
 			scratch.params_offset = g.offset;
 		}
-		uint32_t tc = (t.p1 == 2 || t.p1 == 4 || t.p1 == 8) ? static_cast<uint32_t>(t.p1) : 4u;
+		uint32_t tc = (t.p1 == 2 || t.p1 == 4 || t.p1 == 8) ? static_cast<uint32_t>(t.p1) : 8u;
 		const size_t budget = 150 * 1024;
 		while (tc > 2 && 2ull * tc * scratch.wcap * esz > budget) tc /= 2;
 		if (scratch.max_slots > 511 || 2ull * tc * scratch.wcap * esz > budget) tiled = false;

@@ -1,0 +1,172 @@
+// Command-line STA imaging driver over the C++ processor adapter.
+//
+// Replays the reference's STAMethod call sequence (STAMethod.cpp:120-263 and
+// :96-118): configuration -> image grid -> rx apodization -> coherence
+// factor -> processor.prepare(base) -> processor.process(grid) -> optional
+// coherence-factor image (Util::applyFactorToValue, Util.h:1052-1060).
+// Echoes come from a raw float32 file [ntx][nrx][ns] (what the reference's
+// SavedSTAAcquisition replays per transmit, SavedSTAAcquisition.h:78-90).
+//
+//   mista_sta_cli --echoes FILE --elements N --samples NS --up U
+//                 --pitch M --fs HZ --c MPS --fc HZ
+//                 --grid N1 N2 XMIN XMAX ZMIN ZMAX
+//                 [--cf none|pcf|scf] [--cf-param P] [--apod hanning|rectangular]
+//                 [--storage f32|f16|bf16] [--out PREFIX]
+//
+// Writes PREFIX_value.f32 and PREFIX_factor.f32 ([N1][N2], z fastest) and,
+// with a coherence factor, PREFIX_cf.f32 (value * factor).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "mista/HipSTAProcessor.h"
+
+namespace {
+
+// Matrix<XYZValueFactor<float>> stand-in: row-major n1 x n2 (Matrix.h:188-191).
+struct Grid {
+	uint32_t n1_ = 0, n2_ = 0;
+	std::vector<mista::XYZValueFactor> pts;
+	size_t n1() const { return n1_; }
+	size_t n2() const { return n2_; }
+	mista::XYZValueFactor& operator()(size_t i, size_t j) { return pts[i * n2_ + j]; }
+	const mista::XYZValueFactor& operator()(size_t i, size_t j) const { return pts[i * n2_ + j]; }
+};
+
+class FileAcquisition : public mista::STAAcquisition {
+public:
+	FileAcquisition(std::vector<float> data, unsigned int nrx, unsigned int ns) : data_(std::move(data)), nrx_(nrx), ns_(ns) {}
+	void execute(unsigned int txElem, std::vector<float>& rx, unsigned int& numSamples) override
+	{
+		const size_t n = static_cast<size_t>(nrx_) * ns_;
+		if ((txElem + 1) * n > data_.size()) throw mista::InvalidParameterException("transmit out of range");
+		rx.assign(data_.begin() + txElem * n, data_.begin() + (txElem + 1) * n);
+		numSamples = ns_;
+	}
+private:
+	std::vector<float> data_;
+	unsigned int nrx_, ns_;
+};
+
+// WindowFunction.h:172-183 (hanning: zero endpoints) and rectangular.
+std::vector<float> window(const std::string& name, unsigned int n)
+{
+	std::vector<float> w(n, 1.0f);
+	if (name == "hanning") {
+		for (unsigned int i = 0; i < n; ++i) w[i] = static_cast<float>(0.5 - 0.5 * std::cos(2.0 * M_PI * (i / (n - 1.0))));
+	} else if (name != "rectangular") {
+		throw mista::InvalidParameterException("Invalid window function name: " + name);
+	}
+	return w;
+}
+
+bool write_f32(const std::string& path, const std::vector<float>& v)
+{
+	std::ofstream f(path, std::ios::binary);
+	f.write(reinterpret_cast<const char*>(v.data()), static_cast<std::streamsize>(v.size() * sizeof(float)));
+	return static_cast<bool>(f);
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+	std::string echoes_path, out = "sta", cf = "none", apod = "hanning", storage = "f32";
+	unsigned int N = 0, ns = 0, up = 1, n1 = 0, n2 = 0;
+	float pitch = 0, fs = 0, c = 1540.0f, fc = 0, cf_param = 1.0f;
+	double xmin = 0, xmax = 0, zmin = 0, zmax = 0;
+	for (int i = 1; i < argc; ++i) {
+		const std::string a = argv[i];
+		auto next = [&]() -> const char* {
+			if (i + 1 >= argc) { std::cerr << "missing value for " << a << "\n"; std::exit(2); }
+			return argv[++i];
+		};
+		if (a == "--echoes") echoes_path = next();
+		else if (a == "--elements") N = std::atoi(next());
+		else if (a == "--samples") ns = std::atoi(next());
+		else if (a == "--up") up = std::atoi(next());
+		else if (a == "--pitch") pitch = std::atof(next());
+		else if (a == "--fs") fs = std::atof(next());
+		else if (a == "--c") c = std::atof(next());
+		else if (a == "--fc") fc = std::atof(next());
+		else if (a == "--cf") cf = next();
+		else if (a == "--cf-param") cf_param = std::atof(next());
+		else if (a == "--apod") apod = next();
+		else if (a == "--storage") storage = next();
+		else if (a == "--out") out = next();
+		else if (a == "--grid") {
+			n1 = std::atoi(next()); n2 = std::atoi(next());
+			xmin = std::atof(next()); xmax = std::atof(next()); zmin = std::atof(next()); zmax = std::atof(next());
+		} else { std::cerr << "unknown argument " << a << "\n"; return 2; }
+	}
+	if (echoes_path.empty() || N == 0 || ns == 0 || n1 == 0 || n2 == 0) {
+		std::cerr << "usage: see the header of sta_cli.cpp\n";
+		return 2;
+	}
+	try {
+		std::ifstream f(echoes_path, std::ios::binary | std::ios::ate);
+		if (!f) throw mista::InvalidParameterException("cannot open " + echoes_path);
+		const size_t bytes = static_cast<size_t>(f.tellg());
+		const size_t per_tx = static_cast<size_t>(N) * ns * sizeof(float);
+		if (bytes % per_tx != 0) throw mista::InvalidParameterException("echo file size is not a multiple of one transmit");
+		std::vector<float> data(bytes / sizeof(float));
+		f.seekg(0);
+		f.read(reinterpret_cast<char*>(data.data()), static_cast<std::streamsize>(bytes));
+		const unsigned int ntx = static_cast<unsigned int>(bytes / per_tx);
+
+		mista::STAConfiguration config;
+		config.numElements = N;
+		config.firstTxElem = 0;
+		config.lastTxElem = ntx - 1;
+		config.pitch = pitch;
+		config.samplingFrequency = fs;
+		config.propagationSpeed = c;
+		config.centerFrequency = fc;
+		mista::CoherenceFactor coherence;
+		coherence.kind = cf == "pcf" ? MISTA_CF_PCF : cf == "scf" ? MISTA_CF_SCF : MISTA_CF_NONE;
+		coherence.param = cf_param;
+		const int st = storage == "f16" ? MISTA_STORAGE_F16 : storage == "bf16" ? MISTA_STORAGE_BF16 : MISTA_STORAGE_F32;
+
+		// ImageGrid (explicit point counts, Util.h:326-345 spacing)
+		Grid grid;
+		grid.n1_ = n1;
+		grid.n2_ = n2;
+		grid.pts.resize(static_cast<size_t>(n1) * n2);
+		for (unsigned int i = 0; i < n1; ++i)
+			for (unsigned int j = 0; j < n2; ++j) {
+				grid(i, j).x = static_cast<float>(n1 > 1 ? xmin + i * ((xmax - xmin) / (n1 - 1)) : xmin);
+				grid(i, j).z = static_cast<float>(n2 > 1 ? zmin + j * ((zmax - zmin) / (n2 - 1)) : zmin);
+			}
+
+		FileAcquisition acq(std::move(data), N, ns);
+		mista::HipSTAProcessor<> proc(config, acq, up, coherence, 0.0f, window(apod, N), st);
+		proc.prepare(0);
+		const auto t0 = std::chrono::steady_clock::now();
+		proc.process(grid);
+		const auto t1 = std::chrono::steady_clock::now();
+
+		std::vector<float> value(grid.pts.size()), factor(grid.pts.size()), cfimg(grid.pts.size());
+		for (size_t i = 0; i < grid.pts.size(); ++i) {
+			value[i] = grid.pts[i].value;
+			factor[i] = grid.pts[i].factor;
+			cfimg[i] = grid.pts[i].value * grid.pts[i].factor;   // Util::applyFactorToValue
+		}
+		if (!write_f32(out + "_value.f32", value) || !write_f32(out + "_factor.f32", factor) ||
+				(coherence.enabled() && !write_f32(out + "_cf.f32", cfimg))) {
+			std::cerr << "cannot write outputs\n";
+			return 1;
+		}
+		std::printf("frame: %u tx x %u rx x %u samples, grid %u x %u, %.3f ms (host path incl. transfers)\n", ntx, N, ns, n1,
+				n2, std::chrono::duration<double, std::milli>(t1 - t0).count());
+	} catch (const std::exception& e) {
+		std::cerr << "error: " << e.what() << "\n";
+		return 1;
+	}
+	return 0;
+}
