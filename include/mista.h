@@ -115,8 +115,50 @@ int mista_set_tx_apod(mista_handle* h, const float* tx_apod, uint32_t n);
 int mista_set_grid(mista_handle* h, const float* grid_xz, uint32_t n1, uint32_t n2);
 
 /* Same, for a caller holding the reference's AoS points {x,y,z,value,factor}
- * (5 floats per point, n1*n2 points). */
+ * (5 floats per point, n1*n2 points).  A 2-D processor ignores y, as
+ * Geometry::distance2DY0 does (Geometry.h:55-65). */
 int mista_set_grid_points(mista_handle* h, const float* points, uint32_t n1, uint32_t n2);
+
+/* Grid of n1 x n2 points given as [n1][n2][3] (x, y, z) in m: any
+ * Matrix<XYZValueFactor<float>> layout, as Vectorial3DSTAProcessor::process
+ * takes (Vectorial3DSTAProcessor.h:137-253).  ny > 1 declares a volume laid
+ * out as n1 = nx * ny columns (y fastest) by n2 = nz rows, which lets the DAS
+ * kernel tile it in 3-D; ny = 1 for any other point set.  n1 % ny == 0. */
+int mista_set_grid_xyz(mista_handle* h, const float* grid_xyz, uint32_t n1, uint32_t n2, uint32_t ny);
+
+/*
+ * 3-D sparse STA: replaces SA3DConfiguration<float> (SA3DConfiguration.h:
+ * 37-58) + the constructor arguments of Vectorial3DSTAProcessor
+ * (Vectorial3DSTAProcessor.h:54-60,103-130).  The caller resolves the active
+ * element lists: tx_xy[num_tx][2] are the positions
+ * txElemPos[baseElement + activeTxElem[i]], rx_xy[num_rx][2] the positions
+ * rxElemPos[baseElement + activeRxElem[i]] (ArrayUtil.h:51-69), in m, on
+ * the z = 0 plane.  Echoes are [num_tx][num_rx][ns]; rx apodization has
+ * num_rx entries; distances are Geometry::distance3DZ0 (Geometry.h:77-84).
+ * Every other entry point (grid, apodization, process, device paths) works on
+ * a 3-D handle unchanged.
+ */
+typedef struct mista_sta3d_config {
+	uint32_t num_tx;             /* activeTxElem.size()                          */
+	uint32_t num_rx;             /* activeRxElem.size()                          */
+	uint32_t upsampling_factor;  /* 1..64                                        */
+	float sampling_frequency;    /* Hz                                           */
+	float propagation_speed;     /* m/s                                          */
+	float center_frequency;      /* Hz                                           */
+	float dead_zone_m;           /* m                                            */
+	float peak_offset;           /* periods of center_frequency                  */
+	int32_t cf_kind;             /* MISTA_CF_*                                   */
+	float cf_param;              /* gamma (PCF) or p (SCF)                       */
+	int32_t storage;             /* MISTA_STORAGE_*                              */
+	int32_t device;              /* HIP device ordinal                           */
+} mista_sta3d_config;
+
+int mista_create_3d(const mista_sta3d_config* config, const float* tx_xy, const float* rx_xy, mista_handle** out);
+
+/* New active-element positions for a 3-D handle (a new baseElement in the
+ * reference's prepare, Vectorial3DSTAProcessor.h:133-138); rebuilds the
+ * resident delay table if a grid is set. */
+int mista_set_elements_3d(mista_handle* h, const float* tx_xy, const float* rx_xy);
 
 /* VectorialSTAProcessor::process (VectorialSTAProcessor.h:158-341) for one
  * frame from host memory: echoes [ntx][nrx][ns] float32 (the stacked

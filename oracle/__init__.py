@@ -123,6 +123,9 @@ def lib():
         L.oracle_prepare3d_f32.argtypes = [P3, _f32p, _f32p, i32]
         L.oracle_das3d_f32.restype = i32
         L.oracle_das3d_f32.argtypes = [P3, _f32p, u32, _f32p, _f32p, _f32p, u32, u32, _f32p, _f32p, _f32p, i32]
+        L.oracle_das3d_rxsums_f32.restype = i32
+        L.oracle_das3d_rxsums_f32.argtypes = [P3, _f32p, u32, _f32p, _f32p, _f32p, u32, u32, _f32p, _f32p, _f32p,
+                                              _f32p, i32]
         L.oracle_das3d_f64.restype = i32
         L.oracle_das3d_f64.argtypes = [P3, _f64p, u32, _f32p, _f32p, _f32p, u32, u32, _f32p, _f64p, _f64p, i32]
         _lib = L
@@ -394,3 +397,19 @@ def das3d(cfg, analytic_ri, tx_xy, rx_xy, grid_xyz, rx_apod, nthreads=0):
     _check(fn(ctypes.byref(cfg), analytic_ri, L, tx_xy, rx_xy, grid_xyz, n1, n2, rx_apod,
               value, factor, nthreads))
     return value, factor
+
+
+def das3d_rxsums(cfg, analytic_ri, tx_xy, rx_xy, grid_xyz, rx_apod, nthreads=0):
+    """FP32 3-D DAS also returning the per-rx sums (complex [n1][n2][num_rx])."""
+    analytic_ri = np.ascontiguousarray(analytic_ri, np.float32)
+    L = analytic_ri.shape[2]
+    grid_xyz = np.ascontiguousarray(grid_xyz, np.float32)
+    n1, n2 = grid_xyz.shape[:2]
+    value = np.empty((n1, n2), np.float32)
+    factor = np.empty((n1, n2), np.float32)
+    sums = np.empty((n1, n2, cfg.num_rx, 2), np.float32)
+    _check(lib().oracle_das3d_rxsums_f32(ctypes.byref(cfg), analytic_ri, L,
+                                         np.ascontiguousarray(tx_xy, np.float32),
+                                         np.ascontiguousarray(rx_xy, np.float32), grid_xyz, n1, n2,
+                                         np.ascontiguousarray(rx_apod, np.float32), value, factor, sums, nthreads))
+    return value, factor, sums[..., 0] + 1j * sums[..., 1]

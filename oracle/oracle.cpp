@@ -528,7 +528,7 @@ void das(const oracle_sta2d& cfg, const T* analytic_ri, uint32_t L,
 template<typename T>
 void das3d(const oracle_sta3d& cfg, const T* analytic_ri, uint32_t L,
 		const float* tx_xy, const float* rx_xy, const float* grid_xyz, uint32_t n1, uint32_t n2,
-		const float* rx_apod_f, T* value, T* factor, int nthreads)
+		const float* rx_apod_f, T* value, T* factor, int nthreads, T* rxsum_out = nullptr)
 {
 	using C = std::complex<T>;
 	const C* tensor = reinterpret_cast<const C*>(analytic_ri);
@@ -576,6 +576,12 @@ void das3d(const oracle_sta3d& cfg, const T* analytic_ri, uint32_t L,
 							const C* p = tensor + (static_cast<size_t>(t) * nrx + r) * L + idx;
 							rx_sum[r] += ((1 - k) * *p + k * *(p + 1)) * rx_apod[r];
 						}
+					}
+				}
+				if (rxsum_out) {
+					for (unsigned r = 0; r < nrx; ++r) {
+						rxsum_out[(pt * nrx + r) * 2] = rx_sum[r].real();
+						rxsum_out[(pt * nrx + r) * 2 + 1] = rx_sum[r].imag();
 					}
 				}
 				if (cf == ORACLE_CF_PCF) {
@@ -851,6 +857,15 @@ int oracle_das3d_f32(const oracle_sta3d* cfg, const float* analytic, uint32_t L,
 {
 	return guarded([&] {
 		das3d<float>(*cfg, analytic, L, tx_xy, rx_xy, grid_xyz, n1, n2, rx_apod, value, factor, nthreads);
+	});
+}
+
+int oracle_das3d_rxsums_f32(const oracle_sta3d* cfg, const float* analytic, uint32_t L,
+		const float* tx_xy, const float* rx_xy, const float* grid_xyz, uint32_t n1, uint32_t n2,
+		const float* rx_apod, float* value, float* factor, float* rxsums, int nthreads)
+{
+	return guarded([&] {
+		das3d<float>(*cfg, analytic, L, tx_xy, rx_xy, grid_xyz, n1, n2, rx_apod, value, factor, nthreads, rxsums);
 	});
 }
 

@@ -130,6 +130,10 @@ int oracle_das3d_f32(const oracle_sta3d* cfg, const float* analytic, uint32_t si
 		const float* tx_xy, const float* rx_xy,
 		const float* grid_xyz, uint32_t n1, uint32_t n2,
 		const float* rx_apod, float* value, float* factor, int nthreads);
+int oracle_das3d_rxsums_f32(const oracle_sta3d* cfg, const float* analytic, uint32_t signal_length,
+		const float* tx_xy, const float* rx_xy,
+		const float* grid_xyz, uint32_t n1, uint32_t n2,
+		const float* rx_apod, float* value, float* factor, float* rxsums, int nthreads);
 int oracle_das3d_f64(const oracle_sta3d* cfg, const double* analytic, uint32_t signal_length,
 		const float* tx_xy, const float* rx_xy,
 		const float* grid_xyz, uint32_t n1, uint32_t n2,

@@ -1,0 +1,161 @@
+"""GPU parity of the 3-D sparse STA path (Vectorial3DSTAProcessor semantics,
+Vectorial3DSTAProcessor.h:137-253) through the C ABI against the CPU oracle.
+
+Small volumes: the whole frame (prep + DAS + coherence factor) against
+oracle.prepare3d + oracle.das3d, value within 1e-4 of the image peak, CF
+image to the same bar except at pixels the oracle's per-rx sums show to sit
+on a coherence-factor discontinuity.  Full size (1024 rx x 64 tx x 4096
+samples, 128 x 128 x 256 voxels): prep of one transmit against the oracle,
+and DAS of every voxel of the columns through the brightest voxel against the
+oracle run on the GPU's own analytic tensor."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from us_lab4a_amd import imaging, workloads
+from us_lab4a_amd.sta import HipSTA3DProcessor
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def make_proc(wl, storage="f32", cf=None):
+    cfg = wl.sa3d_config()
+    coh = wl.coherence()
+    if cf is not None:
+        coh.method = cf
+    p = HipSTA3DProcessor(cfg, None, wl.upsampling, coh, wl.peak_offset, wl.rx_apodization(), storage=storage)
+    p.set_grid(wl.grid_xyz(), ny=wl.n_y)
+    return p
+
+
+def oracle_frame(wl, echoes, cf):
+    cfg = wl.oracle_cfg()
+    cfg.cf_kind = {"none": oracle.CF_NONE, "phase_coherence_factor": oracle.CF_PCF,
+                   "sign_coherence_factor": oracle.CF_SCF}[cf]
+    a = oracle.prepare3d(cfg, echoes)
+    tx, rx = wl.sa3d_config().positions(0)
+    v, f, sums = oracle.das3d_rxsums(cfg, a, tx, rx, wl.grid_xyz(), wl.rx_apodization())
+    return a, v, f, sums
+
+
+@pytest.fixture(scope="module")
+def small():
+    wl = workloads.sta3d_small()
+    return wl, wl.echoes_cpu()
+
+
+def test_delay_table_3d_fp64_rounded_once(small):
+    wl, _ = small
+    say("3-D delay table", wl.name)
+    p = make_proc(wl)
+    d = p.download_delays()
+    p.close()
+    g = wl.grid_xyz().reshape(-1, 3).astype(np.float64)
+    tx, rx = wl.sa3d_config().positions(0)
+    el = np.concatenate([rx, tx]).astype(np.float64)     # rows: active rx, then active tx
+    inv = np.float64(np.float32(wl.fs)) * wl.upsampling / np.float64(np.float32(wl.c))
+    ref = np.sqrt((g[None, :, 0] - el[:, None, 0]) ** 2 + (g[None, :, 1] - el[:, None, 1]) ** 2
+                  + g[None, :, 2] ** 2) * inv
+    ulp = np.spacing(ref.astype(np.float32)).astype(np.float64)
+    assert d.shape == ref.shape
+    assert np.all(np.abs(d.astype(np.float64) - ref) <= 0.5 * ulp + 1e-9)
+
+
+@pytest.mark.parametrize("cf", ["none", "sign_coherence_factor", "phase_coherence_factor"])
+def test_small_volume_matches_oracle(small, cf):
+    wl, echoes = small
+    say("3-D small frame", cf)
+    p = make_proc(wl, cf=cf)
+    v, f = p.process_echoes(echoes)
+    a_gpu = p.download_analytic()
+    p.close()
+    a_o, v_o, f_o, sums = oracle_frame(wl, echoes, cf)
+    peak_a = float(np.abs(a_o).max())
+    assert float(np.abs(a_gpu - a_o).max()) <= 2e-6 * peak_a
+    peak = float(np.abs(v_o).max())
+    err = float(np.abs(v.astype(np.float64) - v_o).max()) / peak
+    say("value error / peak", err)
+    assert err <= TOL
+    if cf == "none":
+        assert np.all(f == 1.0)
+        return
+    kind = oracle.CF_SCF if cf == "sign_coherence_factor" else oracle.CF_PCF
+    bad = np.abs(v.astype(np.float64) * f - v_o.astype(np.float64) * f_o) > TOL * peak
+    sens = oracle.branch_sensitive(sums, kind)
+    say("CF image over the bar at", int(bad.sum()), "of", bad.size, "; branch-sensitive", int(sens.sum()))
+    assert not np.any(bad & ~sens)
+    assert bad.mean() <= 0.01
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_volume_tiles_match_gather(small, storage):
+    """The 8x8x8 voxel-tile kernel and the one-voxel-per-lane gather kernel
+    form bit-identical volumes (same per-pair arithmetic, same order)."""
+    wl, echoes = small
+    wl = workloads.sta3d_small(n=(16, 16, 40))
+    echoes = wl.echoes_cpu()
+    say("3-D tiles vs gather", storage)
+    p = make_proc(wl, storage=storage)
+    p.set_kernel(2, 4)
+    v_t, f_t = p.process_echoes(echoes)
+    p.set_kernel(1)
+    v_g, f_g = p.process_echoes(echoes)
+    p.close()
+    np.testing.assert_array_equal(v_t, v_g)
+    np.testing.assert_array_equal(f_t, f_g)
+
+
+def test_full_size_volume(tmp_path):
+    """1024 rx x 64 tx x 4096 samples, U = 4, SCF, 128 x 128 x 256 voxels."""
+    import torch
+    wl = workloads.sta3d_sparse()
+    say("3-D full size", wl.name, wl.pixel_pairs / 1e9, "G voxel*pairs")
+    dev = torch.device("cuda:0")
+    d_echoes = torch.empty((wl.num_tx, wl.num_rx, wl.num_samples), dtype=torch.float32, device=dev)
+    wl.echoes_gpu(d_echoes)
+    npix = wl.n_x * wl.n_y * wl.n_z
+    d_v = torch.empty(npix, dtype=torch.float32, device=dev)
+    d_f = torch.empty(npix, dtype=torch.float32, device=dev)
+    p = make_proc(wl)
+    p.process_device(d_echoes, wl.num_samples, d_v, d_f)
+    torch.cuda.synchronize()
+    prep_ms, das_ms = p.last_timing()
+    say(f"prep {prep_ms:.1f} ms, das {das_ms:.1f} ms")
+    v = d_v.cpu().numpy().reshape(wl.n_x * wl.n_y, wl.n_z)
+    f = d_f.cpu().numpy().reshape(wl.n_x * wl.n_y, wl.n_z)
+    echoes0 = d_echoes[:1].cpu().numpy()
+    del d_echoes
+    a = p.download_analytic()
+    p.close()
+    # prep of transmit 0 (1024 channels) against the oracle
+    cfg1 = wl.oracle_cfg()
+    cfg1.num_tx = 1
+    a_o = oracle.prepare3d(cfg1, echoes0)
+    assert float(np.abs(a[:1] - a_o).max()) <= 2e-6 * float(np.abs(a_o).max())
+    del a_o
+    # DAS of the z-columns through the brightest voxel and 7 others
+    col_peak = int(np.unravel_index(np.argmax(v), v.shape)[0])
+    cols = np.unique(np.concatenate([[col_peak], np.linspace(0, v.shape[0] - 1, 7).astype(int)]))
+    g = np.ascontiguousarray(wl.grid_xyz()[cols])
+    tx, rx = wl.sa3d_config().positions(0)
+    cfg = wl.oracle_cfg()
+    v_o, f_o, sums = oracle.das3d_rxsums(cfg, a, tx, rx, g, wl.rx_apodization())
+    peak = float(np.abs(v).max())
+    err = float(np.abs(v[cols].astype(np.float64) - v_o).max()) / peak
+    say("columns", cols.tolist(), "value error / peak", err)
+    assert abs(float(np.abs(v_o).max()) - peak) <= TOL * peak   # the subset holds the volume peak
+    assert err <= TOL
+    bad = np.abs(v[cols].astype(np.float64) * f[cols] - v_o.astype(np.float64) * f_o) > TOL * peak
+    sens = oracle.branch_sensitive(sums, oracle.CF_SCF)
+    say("CF over the bar", int(bad.sum()), "branch-sensitive", int(sens.sum()))
+    assert not np.any(bad & ~sens)
+    assert bad.mean() <= 0.01

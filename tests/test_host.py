@@ -69,10 +69,41 @@ def test_create_rejects_invalid_configuration(L, kw, code):
     assert L.mista_last_error(None)
 
 
+def _cfg3d(**kw):
+    c = _lib.Sta3dConfig()
+    base = dict(num_tx=4, num_rx=16, upsampling_factor=4, sampling_frequency=40e6, propagation_speed=1540.0,
+                center_frequency=3e6, dead_zone_m=0.0, peak_offset=0.0, cf_kind=_lib.CF_SCF, cf_param=1.0,
+                storage=_lib.STORAGE_F32, device=0)
+    base.update(kw)
+    for k, v in base.items():
+        setattr(c, k, v)
+    return c
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(num_tx=0), _lib.MISTA_E_INVALID_VALUE),           # SA3DConfiguration: empty active list
+    (dict(num_rx=0), _lib.MISTA_E_INVALID_VALUE),
+    (dict(upsampling_factor=0), _lib.MISTA_E_INVALID_VALUE),
+    (dict(sampling_frequency=0.0), _lib.MISTA_E_INVALID_PARAM),
+    (dict(cf_kind=7), _lib.MISTA_E_INVALID_PARAM),
+    (dict(storage=9), _lib.MISTA_E_INVALID_PARAM),
+])
+def test_create_3d_rejects_invalid_configuration(L, kw, code):
+    h = ctypes.c_void_p()
+    xy = np.zeros((64, 2), np.float32)
+    rc = L.mista_create_3d(ctypes.byref(_cfg3d(**kw)), xy.ctypes.data, xy.ctypes.data, ctypes.byref(h))
+    assert rc == code
+    assert h.value is None
+    assert L.mista_last_error(None)
+
+
 def test_null_handle_calls_fail_cleanly(L):
     assert L.mista_destroy(None) == 0
     assert L.mista_set_grid(None, None, 1, 1) == _lib.MISTA_E_INVALID_PARAM
     assert L.mista_process(None, None, 1, None, None) == _lib.MISTA_E_INVALID_PARAM
+    assert L.mista_set_grid_xyz(None, None, 1, 1, 1) == _lib.MISTA_E_INVALID_PARAM
+    assert L.mista_set_elements_3d(None, None, None) == _lib.MISTA_E_INVALID_PARAM
+    assert L.mista_create_3d(None, None, None, None) == _lib.MISTA_E_INVALID_PARAM
 
 
 def test_windows_match_oracle():

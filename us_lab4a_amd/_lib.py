@@ -41,6 +41,24 @@ class StaConfig(ctypes.Structure):
     ]
 
 
+class Sta3dConfig(ctypes.Structure):
+    """mista_sta3d_config (include/mista.h)."""
+    _fields_ = [
+        ("num_tx", ctypes.c_uint32),
+        ("num_rx", ctypes.c_uint32),
+        ("upsampling_factor", ctypes.c_uint32),
+        ("sampling_frequency", ctypes.c_float),
+        ("propagation_speed", ctypes.c_float),
+        ("center_frequency", ctypes.c_float),
+        ("dead_zone_m", ctypes.c_float),
+        ("peak_offset", ctypes.c_float),
+        ("cf_kind", ctypes.c_int32),
+        ("cf_param", ctypes.c_float),
+        ("storage", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+    ]
+
+
 class SynthParams(ctypes.Structure):
     """mista_synth_params (include/mista.h)."""
     _fields_ = [
@@ -101,6 +119,9 @@ SIGNATURES = {
     "mista_set_tx_apod": (_i32, [_vp, _vp, _u32]),
     "mista_set_grid": (_i32, [_vp, _vp, _u32, _u32]),
     "mista_set_grid_points": (_i32, [_vp, _vp, _u32, _u32]),
+    "mista_set_grid_xyz": (_i32, [_vp, _vp, _u32, _u32, _u32]),
+    "mista_create_3d": (_i32, [ctypes.POINTER(Sta3dConfig), _vp, _vp, ctypes.POINTER(_vp)]),
+    "mista_set_elements_3d": (_i32, [_vp, _vp, _vp]),
     "mista_process": (_i32, [_vp, _vp, _u32, _vp, _vp]),
     "mista_process_points": (_i32, [_vp, _vp, _u32, _vp]),
     "mista_process_device": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp]),

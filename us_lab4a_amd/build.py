@@ -12,6 +12,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -21,7 +22,8 @@ HOST_DRIVER = os.path.join(PKG, "mista_sta_cli")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["api.cpp", "prep.hip", "das.hip", "synth.hip", "serve.cpp"]
+SOURCES = ["api.cpp", "prep.hip", "das.hip", "das_tile_f32.hip", "das_tile_f16.hip", "das_tile_bf16.hip",
+           "synth.hip", "serve.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
@@ -45,15 +47,19 @@ def build_library(verbose=True):
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     build_dir = os.path.join(PKG, "_build")
     os.makedirs(build_dir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in srcs:
         obj = os.path.join(build_dir, os.path.basename(src) + ".o")
         lang = ["-x", "hip"]
-        cmd = [HIPCC] + HIP_FLAGS + lang + ["-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
+        cmds.append([HIPCC] + HIP_FLAGS + lang + ["-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj])
+        objs.append(obj)
         if verbose:
             print("[build]", os.path.relpath(src, ROOT), flush=True)
-        _run(cmd)
-        objs.append(obj)
+    # translation units compile in parallel (the tiled DAS kernels dominate)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0")) or os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs +
          ["-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"])

@@ -32,9 +32,17 @@ struct mista_handle {
 	DeviceBuffer d_rx_apod, d_tx_apod, d_w;
 	bool weights_dirty = true;
 
+	// delay-table rows: 2-D one per array element (tx t at row first_tx + t);
+	// 3-D active rx elements, then active tx elements (positions in elem_xy)
+	bool is3d = false;
+	uint32_t rows = 0, tx_row0 = 0;
+	std::vector<double> elem_xy;   // 3-D: [rows][2]
+	DeviceBuffer d_elem;
+
 	// grid + resident delay table
-	uint32_t n1 = 0, n2 = 0;
+	uint32_t n1 = 0, n2 = 0, ny = 1;
 	uint64_t grid_version = 0;
+	std::vector<float> grid_xyz;   // host copy, [n1*n2][3]
 	DeviceBuffer d_grid, d_delay;
 
 	// frame state
@@ -124,8 +132,8 @@ void set_frame_geom(mista_handle* h, uint32_t ns, uint32_t L, hipStream_t s)
 {
 	const mista_sta_config& c = h->cfg;
 	FrameGeom& g = h->geom;
-	g.num_elements = c.num_elements;
-	g.first_tx = c.first_tx_elem;
+	g.num_elements = h->rows;
+	g.first_tx = h->tx_row0;
 	g.ntx = ntx_of(c);
 	g.ntx_pad = pad_ntx(g.ntx);
 	g.nrx = c.num_elements;
@@ -139,6 +147,7 @@ void set_frame_geom(mista_handle* h, uint32_t ns, uint32_t L, hipStream_t s)
 	g.nf = hilbert_fft_size(L);
 	g.n1 = h->n1;
 	g.n2 = h->n2;
+	g.ny = h->ny;
 	g.npix = static_cast<uint64_t>(h->n1) * h->n2;
 	g.grid_version = h->grid_version;
 	// VectorialSTAProcessor.h:147 (float): (fs*U) * peakOffset / fc
@@ -226,6 +235,7 @@ void run_das(mista_handle* h, float* d_value, float* d_factor, hipStream_t s)
 	// the grid may have changed since the signals were prepared
 	h->geom.n1 = h->n1;
 	h->geom.n2 = h->n2;
+	h->geom.ny = h->ny;
 	h->geom.npix = static_cast<uint64_t>(h->n1) * h->n2;
 	h->geom.grid_version = h->grid_version;
 	upload_apod(h, s);
@@ -238,6 +248,95 @@ void run_das(mista_handle* h, float* d_value, float* d_factor, hipStream_t s)
 			d_value, d_factor, s);
 	HIP_CHECK(hipEventRecord(h->ev[3], s));
 	h->timed = true;
+}
+
+// Builds the resident delay table from h->grid_xyz and the element rows.
+void build_delays(mista_handle* h)
+{
+	hipStream_t s = h->stream;
+	const uint64_t npix = static_cast<uint64_t>(h->n1) * h->n2;
+	const double inv_ct = (static_cast<double>(h->cfg.sampling_frequency) * h->cfg.upsampling_factor) /
+			static_cast<double>(h->cfg.propagation_speed);
+	float* d_delay = static_cast<float*>(h->d_delay.reserve(npix * h->rows * sizeof(float)));
+	if (h->is3d) {
+		void* d_grid = h->d_grid.reserve(npix * 3 * sizeof(float));
+		HIP_CHECK(hipMemcpyAsync(d_grid, h->grid_xyz.data(), npix * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+		void* d_elem = h->d_elem.reserve(h->elem_xy.size() * sizeof(double));
+		HIP_CHECK(hipMemcpyAsync(d_elem, h->elem_xy.data(), h->elem_xy.size() * sizeof(double), hipMemcpyHostToDevice, s));
+		launch_delay_table_3d(static_cast<const float*>(d_grid), npix, static_cast<const double*>(d_elem), h->rows, inv_ct,
+				d_delay, s);
+	} else {
+		// Geometry::distance2DY0 (Geometry.h:55-65): y of the points is not used
+		std::vector<float> xz(npix * 2);
+		for (uint64_t i = 0; i < npix; ++i) {
+			xz[2 * i] = h->grid_xyz[3 * i];
+			xz[2 * i + 1] = h->grid_xyz[3 * i + 2];
+		}
+		void* d_grid = h->d_grid.reserve(npix * 2 * sizeof(float));
+		HIP_CHECK(hipMemcpyAsync(d_grid, xz.data(), npix * 2 * sizeof(float), hipMemcpyHostToDevice, s));
+		launch_delay_table(static_cast<const float*>(d_grid), npix, h->rows, static_cast<double>(h->cfg.pitch), inv_ct,
+				d_delay, s);
+	}
+	HIP_CHECK(hipStreamSynchronize(s));
+	++h->grid_version;
+}
+
+// Validates and installs a grid of n1 x n2 points (x, y, z); ny > 1 marks a
+// volume laid out as n1 = nx * ny columns, y fastest.
+void set_grid_xyz(mista_handle* h, std::vector<float>&& xyz, uint32_t n1, uint32_t n2, uint32_t ny)
+{
+	if (n1 == 0 || n2 == 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "Empty image grid.");
+	if (ny == 0 || n1 % ny != 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "n1 (" << n1 << ") is not a multiple of ny (" << ny << ").");
+	const uint64_t npix = static_cast<uint64_t>(n1) * n2;
+	if (npix > (1ull << 32)) MISTA_THROW(MISTA_E_INVALID_PARAM, "Too many grid points: " << npix << '.');
+	for (uint64_t i = 0; i < 3 * npix; ++i)
+		if (!std::isfinite(xyz[i]) || std::fabs(xyz[i]) > 1.0e3f)
+			MISTA_THROW(MISTA_E_INVALID_VALUE, "Grid coordinate " << i / 3 << " is not finite or exceeds 1 km.");
+	h->grid_xyz = std::move(xyz);
+	h->n1 = n1;
+	h->n2 = n2;
+	h->ny = ny;
+	try {
+		build_delays(h);
+	} catch (...) {
+		h->n1 = h->n2 = 0;   // no half-built grid
+		throw;
+	}
+}
+
+void validate3d(const mista_sta3d_config& c)
+{
+	if (c.num_tx == 0 || c.num_rx == 0) MISTA_THROW(MISTA_E_INVALID_VALUE, "Empty list of active elements.");
+	if (c.num_tx > 4096 || c.num_rx > 65536) MISTA_THROW(MISTA_E_INVALID_PARAM, "Too many active elements.");
+	if (c.upsampling_factor < 1 || c.upsampling_factor > 64)
+		MISTA_THROW(MISTA_E_INVALID_VALUE, "Invalid upsampling factor: " << c.upsampling_factor << ". Must be in [1, 64].");
+	if (!(c.sampling_frequency > 0) || !(c.propagation_speed > 0) || !(c.center_frequency > 0))
+		MISTA_THROW(MISTA_E_INVALID_PARAM, "sampling_frequency, propagation_speed and center_frequency must be > 0.");
+	if (!(c.dead_zone_m >= 0)) MISTA_THROW(MISTA_E_INVALID_PARAM, "dead_zone_m must be >= 0.");
+	if (!(c.peak_offset >= 0)) MISTA_THROW(MISTA_E_INVALID_PARAM, "peak_offset must be >= 0.");
+	if (c.cf_kind != MISTA_CF_NONE && c.cf_kind != MISTA_CF_PCF && c.cf_kind != MISTA_CF_SCF)
+		MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid coherence factor method: " << c.cf_kind << '.');
+	if (c.cf_kind != MISTA_CF_NONE && !(c.cf_param >= 0 && c.cf_param <= 100))
+		MISTA_THROW(MISTA_E_INVALID_PARAM, "coherence factor parameter out of [0, 100].");
+	if (c.storage != MISTA_STORAGE_F32 && c.storage != MISTA_STORAGE_F16 && c.storage != MISTA_STORAGE_BF16)
+		MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid storage type: " << c.storage << '.');
+}
+
+void set_elements_3d(mista_handle* h, const float* tx_xy, const float* rx_xy)
+{
+	const uint32_t nrx = h->cfg.num_elements, ntx = ntx_of(h->cfg);
+	std::vector<double> e(static_cast<size_t>(nrx + ntx) * 2);
+	for (uint32_t r = 0; r < nrx; ++r) {
+		e[2 * r] = rx_xy[2 * r];
+		e[2 * r + 1] = rx_xy[2 * r + 1];
+	}
+	for (uint32_t t = 0; t < ntx; ++t) {
+		e[2 * (nrx + t)] = tx_xy[2 * t];
+		e[2 * (nrx + t) + 1] = tx_xy[2 * t + 1];
+	}
+	for (double v : e)
+		if (!std::isfinite(v) || std::fabs(v) > 1.0e3) MISTA_THROW(MISTA_E_INVALID_VALUE, "Element position is not finite or exceeds 1 km.");
+	h->elem_xy.swap(e);
 }
 
 }  // namespace
@@ -267,9 +366,62 @@ int mista_create(const mista_sta_config* config, mista_handle** out)
 		HIP_CHECK(hipSetDevice(config->device));
 		auto h = std::make_unique<mista_handle>();
 		h->cfg = *config;
+		h->rows = config->num_elements;
+		h->tx_row0 = config->first_tx_elem;
 		HIP_CHECK(hipStreamCreate(&h->stream));  // default priority
 		h->rx_apod.assign(config->num_elements, 1.0f);
 		*out = h.release();
+	});
+}
+
+int mista_create_3d(const mista_sta3d_config* config, const float* tx_xy, const float* rx_xy, mista_handle** out)
+{
+	if (!config || !out || !tx_xy || !rx_xy) {
+		g_thread_error = "null argument";
+		return MISTA_E_INVALID_PARAM;
+	}
+	*out = nullptr;
+	return guarded(nullptr, [&] {
+		validate3d(*config);
+		int ndev = 0;
+		HIP_CHECK(hipGetDeviceCount(&ndev));
+		if (config->device < 0 || config->device >= ndev)
+			MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid device " << config->device << " (" << ndev << " devices).");
+		HIP_CHECK(hipSetDevice(config->device));
+		auto h = std::make_unique<mista_handle>();
+		mista_sta_config& c = h->cfg;
+		c.num_elements = config->num_rx;   // rx apodization size = active rx count
+		c.first_tx_elem = 0;
+		c.last_tx_elem = config->num_tx - 1;
+		c.upsampling_factor = config->upsampling_factor;
+		c.pitch = 0.0f;
+		c.sampling_frequency = config->sampling_frequency;
+		c.propagation_speed = config->propagation_speed;
+		c.center_frequency = config->center_frequency;
+		c.dead_zone_m = config->dead_zone_m;
+		c.peak_offset = config->peak_offset;
+		c.cf_kind = config->cf_kind;
+		c.cf_param = config->cf_param;
+		c.storage = config->storage;
+		c.device = config->device;
+		h->is3d = true;
+		h->rows = config->num_rx + config->num_tx;
+		h->tx_row0 = config->num_rx;
+		set_elements_3d(h.get(), tx_xy, rx_xy);
+		HIP_CHECK(hipStreamCreate(&h->stream));  // default priority
+		h->rx_apod.assign(config->num_rx, 1.0f);
+		*out = h.release();
+	});
+}
+
+int mista_set_elements_3d(mista_handle* h, const float* tx_xy, const float* rx_xy)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!h->is3d) MISTA_THROW(MISTA_E_INVALID_STATE, "Not a 3-D processor.");
+		if (!tx_xy || !rx_xy) MISTA_THROW(MISTA_E_INVALID_PARAM, "null argument.");
+		set_elements_3d(h, tx_xy, rx_xy);
+		if (h->n1 && h->n2) build_delays(h);
 	});
 }
 
@@ -324,35 +476,37 @@ int mista_set_grid(mista_handle* h, const float* grid_xz, uint32_t n1, uint32_t 
 	return guarded(h, [&] {
 		if (!grid_xz || n1 == 0 || n2 == 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "Empty image grid.");
 		const uint64_t npix = static_cast<uint64_t>(n1) * n2;
-		for (uint64_t i = 0; i < 2 * npix; ++i)
-			if (!std::isfinite(grid_xz[i]) || std::fabs(grid_xz[i]) > 1.0e3f)
-				MISTA_THROW(MISTA_E_INVALID_VALUE, "Grid coordinate " << i / 2 << " is not finite or exceeds 1 km.");
-		hipStream_t s = h->stream;
-		void* d_grid = h->d_grid.reserve(npix * 2 * sizeof(float));
-		HIP_CHECK(hipMemcpyAsync(d_grid, grid_xz, npix * 2 * sizeof(float), hipMemcpyHostToDevice, s));
-		float* d_delay = static_cast<float*>(h->d_delay.reserve(npix * h->cfg.num_elements * sizeof(float)));
-		const double inv_ct = (static_cast<double>(h->cfg.sampling_frequency) * h->cfg.upsampling_factor) /
-				static_cast<double>(h->cfg.propagation_speed);
-		launch_delay_table(static_cast<const float*>(d_grid), npix, h->cfg.num_elements, static_cast<double>(h->cfg.pitch),
-				inv_ct, d_delay, s);
-		HIP_CHECK(hipStreamSynchronize(s));
-		h->n1 = n1;
-		h->n2 = n2;
-		++h->grid_version;
+		std::vector<float> xyz(npix * 3);
+		for (uint64_t i = 0; i < npix; ++i) {
+			xyz[3 * i] = grid_xz[2 * i];
+			xyz[3 * i + 1] = 0.0f;
+			xyz[3 * i + 2] = grid_xz[2 * i + 1];
+		}
+		set_grid_xyz(h, std::move(xyz), n1, n2, 1);
+	});
+}
+
+int mista_set_grid_xyz(mista_handle* h, const float* grid_xyz, uint32_t n1, uint32_t n2, uint32_t ny)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!grid_xyz || n1 == 0 || n2 == 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "Empty image grid.");
+		const uint64_t npix = static_cast<uint64_t>(n1) * n2;
+		set_grid_xyz(h, std::vector<float>(grid_xyz, grid_xyz + 3 * npix), n1, n2, ny);
 	});
 }
 
 int mista_set_grid_points(mista_handle* h, const float* points, uint32_t n1, uint32_t n2)
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
-	if (!points) return guarded(h, [&] { MISTA_THROW(MISTA_E_INVALID_PARAM, "points is null."); });
-	const uint64_t npix = static_cast<uint64_t>(n1) * n2;
-	std::vector<float> xz(npix * 2);
-	for (uint64_t i = 0; i < npix; ++i) {
-		xz[2 * i] = points[5 * i];
-		xz[2 * i + 1] = points[5 * i + 2];
-	}
-	return mista_set_grid(h, xz.data(), n1, n2);
+	return guarded(h, [&] {
+		if (!points || n1 == 0 || n2 == 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "Empty image grid.");
+		const uint64_t npix = static_cast<uint64_t>(n1) * n2;
+		std::vector<float> xyz(npix * 3);
+		for (uint64_t i = 0; i < npix; ++i)
+			for (int k = 0; k < 3; ++k) xyz[3 * i + k] = points[5 * i + k];
+		set_grid_xyz(h, std::move(xyz), n1, n2, 1);
+	});
 }
 
 int mista_process_device(mista_handle* h, const float* d_echoes, uint32_t num_samples, float* d_value, float* d_factor,
@@ -463,7 +617,7 @@ int mista_download_delays(mista_handle* h, float* delays)
 	return guarded(h, [&] {
 		require_grid(h);
 		if (!delays) MISTA_THROW(MISTA_E_INVALID_PARAM, "null host pointer.");
-		const size_t bytes = static_cast<size_t>(h->n1) * h->n2 * h->cfg.num_elements * sizeof(float);
+		const size_t bytes = static_cast<size_t>(h->n1) * h->n2 * h->rows * sizeof(float);
 		HIP_CHECK(hipMemcpyAsync(delays, h->d_delay.get(), bytes, hipMemcpyDeviceToHost, h->stream));
 		HIP_CHECK(hipStreamSynchronize(h->stream));
 	});

@@ -67,10 +67,12 @@ private:
 	size_t cap_ = 0;
 };
 
-// Geometry of one 2-D frame, everything the kernels need by value.
+// Geometry of one frame, everything the kernels need by value.  The delay
+// table has num_elements rows: rx r reads row r, tx t reads row first_tx + t
+// (2-D: one row per array element; 3-D: active rx rows, then active tx rows).
 struct FrameGeom {
-	uint32_t num_elements;
-	uint32_t first_tx;
+	uint32_t num_elements;  // rows of the delay table
+	uint32_t first_tx;      // row of transmit 0
 	uint32_t ntx;
 	uint32_t ntx_pad;   // transmits padded to 32/64/128 (multiple of 8 above)
 	uint32_t nrx;
@@ -80,6 +82,7 @@ struct FrameGeom {
 	uint32_t nf;        // Hilbert FFT size
 	uint32_t dz_up;     // dead zone, upsampled samples
 	uint32_t n1, n2;    // grid
+	uint32_t ny;        // volume grids: n1 = nx * ny with y fastest (1 for a plane)
 	uint64_t npix;
 	float offset;       // signalOffset_ (samples)
 	double inv_ct;      // (fs*U)/c, samples per metre
@@ -114,6 +117,8 @@ inline size_t storage_bytes(int storage) { return storage == MISTA_STORAGE_F32 ?
 
 // ---- kernels launchers (defined in the .hip files) ----
 void launch_delay_table(const float* d_grid_xz, uint64_t npix, uint32_t num_elements, double pitch,
+		double inv_ct, float* d_delay, hipStream_t s);
+void launch_delay_table_3d(const float* d_grid_xyz, uint64_t npix, const double* d_elem_xy, uint32_t rows,
 		double inv_ct, float* d_delay, hipStream_t s);
 
 struct PrepPlan;  // hipFFT plans + scratch for signal preparation

@@ -1,0 +1,314 @@
+// DAS types and the LDS-tiled kernel template, shared by das.hip and the
+// per-storage translation units (das_tile_f32.hip, das_tile_f16.hip,
+// das_tile_bf16.hip) that instantiate the tiled kernel, so the three storage
+// kinds compile in parallel.
+#pragma once
+
+#include "common.h"
+
+namespace mista {
+namespace das {
+
+constexpr float kPi = 3.14159265358979323846f;
+
+template<int STORAGE>
+__device__ __forceinline__ float2 load_sample(const void* __restrict__ t, uint64_t i)
+{
+	if constexpr (STORAGE == MISTA_STORAGE_F32) {
+		return static_cast<const float2*>(t)[i];
+	} else if constexpr (STORAGE == MISTA_STORAGE_F16) {
+		return __half22float2(static_cast<const __half2*>(t)[i]);
+	} else {
+		const uint32_t b = static_cast<const uint32_t*>(t)[i];
+		return make_float2(__uint_as_float(b << 16), __uint_as_float(b & 0xFFFF0000u));
+	}
+}
+
+// LDS entry type per storage kind and its conversion to FP32 complex.
+template<int STORAGE> struct Entry;
+template<> struct Entry<MISTA_STORAGE_F32> {
+	using T = float2;
+	static __device__ __forceinline__ float2 get(T v) { return v; }
+};
+template<> struct Entry<MISTA_STORAGE_F16> {
+	using T = __half2;
+	static __device__ __forceinline__ float2 get(T v) { return __half22float2(v); }
+};
+template<> struct Entry<MISTA_STORAGE_BF16> {
+	using T = uint32_t;
+	static __device__ __forceinline__ float2 get(T b)
+	{
+		return make_float2(__uint_as_float(b << 16), __uint_as_float(b & 0xFFFF0000u));
+	}
+};
+
+struct DasArgs {
+	const void* tensor;   // [ntx][nrx][L] complex, storage type
+	const float* delay;   // [num_elements][npix]
+	const float* w;       // [nrx][ntx]
+	float* value;         // [npix]
+	float* factor;        // [npix] or null
+	uint64_t npix;
+	uint32_t first_tx, ntx, nrx, L;
+	uint32_t w_stride;    // row stride of w (transmits padded)
+	float offset;
+	float cf_factor;      // PCF: gamma / sigma0; SCF: p
+	float inv_num;        // 1 / (Ntx * Nrx)
+};
+
+// Shifted single-pass moments for the two-pass population sigma of
+// Statistics.h:34-58 (shift = first value, accumulated in double).
+struct Moments {
+	double k = 0.0, s1 = 0.0, s2 = 0.0;
+	__device__ void add(float x, bool first)
+	{
+		if (first) k = x;
+		const double d = static_cast<double>(x) - k;
+		s1 += d;
+		s2 += d * d;
+	}
+	__device__ float sigma(uint32_t n) const
+	{
+		const double v = (s2 - s1 * s1 / n) / n;
+		return static_cast<float>(sqrt(v > 0.0 ? v : 0.0));
+	}
+};
+
+// Per-rx statistics and the final value/factor of one pixel.
+template<int CF>
+struct PixelStats {
+	float sr = 0.0f, si = 0.0f;
+	Moments m0, m1;
+	int sign_sum = 0;
+	__device__ __forceinline__ void add_rx(float ar, float ai, bool first)
+	{
+		sr += ar;
+		si += ai;
+		if constexpr (CF == MISTA_CF_PCF) {
+			// PhaseCoherenceFactor::calculate, CoherenceFactor.h:270-288
+			const float phi = atan2f(ai, ar);
+			const float aux = phi + ((phi < 0.0f) ? kPi : -kPi);
+			m0.add(phi, first);
+			m1.add(aux, first);
+		} else if constexpr (CF == MISTA_CF_SCF) {
+			// SignCoherenceFactor::calculate, CoherenceFactor.h:198-215
+			sign_sum += (ar >= 0.0f) ? 1 : -1;
+		}
+	}
+	__device__ __forceinline__ void write(const DasArgs& a, uint64_t p) const
+	{
+		a.value[p] = sqrtf(sr * sr + si * si) * a.inv_num;
+		if (!a.factor) return;
+		float f = 1.0f;
+		if constexpr (CF == MISTA_CF_PCF) {
+			const float sf = fminf(m0.sigma(a.nrx), m1.sigma(a.nrx));
+			f = fmaxf(0.0f, 1.0f - a.cf_factor * sf);
+		} else if constexpr (CF == MISTA_CF_SCF) {
+			const float k = static_cast<float>(sign_sum) / static_cast<float>(a.nrx);
+			const float mm = 1.0f - sqrtf(1.0f - k * k);
+			f = powf(fabsf(mm), a.cf_factor);
+		}
+		a.factor[p] = f;
+	}
+};
+
+// Tile geometry: a tw1 (x) by tw2 (y) by tzh (z) block of grid points, one
+// point per lane, x fastest within a wave (neighbouring lanes read nearby
+// window entries: few LDS conflicts).  A plane has ny = tw2 = 1; a volume is
+// laid out as n1 = nx * ny columns (y fastest) by n2 = nz rows.  Tiles are
+// numbered depth-band major (all tiles of a band of tzh rows are
+// consecutive), so the workgroups resident at one time cover a depth band and
+// share signal windows through L2/MALL.
+struct TileGeom {
+	uint32_t n1, n2, ny, nx;
+	uint32_t tw1, tw2, tzh;
+	uint32_t tiles_x, tiles_y, tiles_z;
+	__device__ __forceinline__ void pixel(uint32_t tile, uint32_t lane_id, uint32_t& col, uint32_t& row, bool& valid) const
+	{
+		const uint32_t per_band = tiles_x * tiles_y;
+		const uint32_t tz = tile / per_band, rem = tile % per_band;
+		const uint32_t tx = rem / tiles_y, ty = rem % tiles_y;
+		const uint32_t lx = lane_id % tw1, ly = (lane_id / tw1) % tw2, lz = lane_id / (tw1 * tw2);
+		const uint32_t ix = tx * tw1 + lx, iy = ty * tw2 + ly, iz = tz * tzh + lz;
+		valid = ix < nx && iy < ny && iz < n2;
+		// lanes outside the grid shadow an in-tile point
+		col = min(ix, nx - 1) * ny + min(iy, ny - 1);
+		row = min(iz, n2 - 1);
+	}
+};
+
+// Window parameters per (tile, rx, tx), packed in a u32:
+//   bits  0..20  ws: first sample of the window (floor of the tile's lowest position)
+//   bits 21..29  number of SLOT-entry load slots to stage
+//   bit  31      record-end flag: some lane may have i + 1 >= L (or pos < 0)
+// lo/hi use the same float expression order as the positions,
+// (offset + d_tx) + d_rx, so by monotonic rounding every lane's floor(pos)
+// lies in [ws, floor(hi)] and floor(pos) + 1 in the staged window.
+struct WinParam {
+	uint32_t start;   // absolute entry index of the window start (channel * L + ws)
+	uint32_t word;    // ws | slots << 21 | edge << 31
+};
+constexpr uint32_t kWsMask = (1u << 21) - 1;
+constexpr uint32_t kEdgeBit = 1u << 31;
+
+typedef __attribute__((address_space(1))) const void* gptr_t;
+typedef __attribute__((address_space(3))) void* lptr_t;
+// Read-only tables (window parameters, weights) through the constant
+// address space: uniform indices then become scalar loads.
+typedef __attribute__((address_space(4))) const WinParam* cu2_t;
+typedef __attribute__((address_space(4))) const float* cf32_t;
+
+struct TileArgs {
+	DasArgs d;
+	TileGeom tg;
+	const WinParam* params;   // [ntiles][nrx][ntx_pad]
+	uint32_t wcap;            // LDS window capacity per pair (entries, multiple of SLOT)
+	uint32_t ntx_pad;         // transmits padded to the instantiated count
+};
+
+template<int STORAGE, int CF, int NTXMAX, int TC, int NT>
+__global__ void __launch_bounds__(NT) das_tile_kernel(const TileArgs A)
+{
+	using E = typename Entry<STORAGE>::T;
+	constexpr uint32_t ESZ = sizeof(E);
+	constexpr uint32_t EPL = 16 / ESZ;      // entries per 16-byte lane load
+	constexpr uint32_t SLOT = 64 * EPL;     // entries per wave load instruction
+	constexpr uint32_t NW = NT / 64;
+	extern __shared__ __attribute__((aligned(16))) unsigned char smem[];   // [2][TC][wcap] entries
+
+	const DasArgs& a = A.d;
+	// XCD-aware order: workgroups are dealt round-robin to the 8 XCDs
+	// (block b runs on XCD b % 8), so give XCD x one contiguous run of tiles;
+	// tiles running at the same time on one XCD then share windows in its L2.
+	const uint32_t ntiles = gridDim.x, q = ntiles / 8, rm = ntiles % 8;
+	const uint32_t xcd = blockIdx.x % 8, tile = xcd * q + min(xcd, rm) + blockIdx.x / 8;
+	const uint32_t tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+	uint32_t col, row;
+	bool valid;
+	A.tg.pixel(tile, tid, col, row, valid);
+	const uint64_t npix = a.npix;
+	const uint64_t p = static_cast<uint64_t>(col) * A.tg.n2 + row;
+	const uint32_t L = a.L, ntx = a.ntx, nrx = a.nrx;
+	const float lim = static_cast<float>(L - 1u);
+	const uint32_t wcap = A.wcap;
+	const cu2_t prm = (cu2_t)(A.params + static_cast<uint64_t>(tile) * nrx * NTXMAX);
+	const unsigned char* tensor = static_cast<const unsigned char*>(a.tensor);
+
+	// tx delays of this lane's pixel, offset folded in: (offset + d_tx)
+	// Transmits are padded to NTXMAX: padded ones have weight 0, an empty
+	// window and a position far below 0, so they take the excluded path.
+	float dtx[NTXMAX];
+#pragma unroll
+	for (int t = 0; t < NTXMAX; ++t)
+		dtx[t] = (static_cast<uint32_t>(t) < ntx) ? a.offset + a.delay[static_cast<uint64_t>(a.first_tx + t) * npix + p] : -1.0e30f;
+
+	// Copy the windows of pairs (r, c*TC .. c*TC+TC-1) into buffer b: wave w
+	// issues load slots w, w + NW, ... of every pair; each lane moves 16 bytes.
+	// Slot sl of pair j is load number j * S + sl of the chunk (S = slots of
+	// the widest window) and goes to wave (j * S + sl) % NW, so the loads of a
+	// chunk spread over all waves.
+	const uint32_t S = wcap / SLOT;
+	auto stage = [&](uint32_t b, uint32_t r, uint32_t c) {
+		const cu2_t pr = prm + r * NTXMAX + c * TC;
+#pragma unroll
+		for (uint32_t j = 0; j < TC; ++j) {
+			const uint32_t slots = (pr[j].word >> 21) & 511u;
+			const unsigned char* src = tensor + static_cast<uint64_t>(pr[j].start) * ESZ + lane * 16;
+			const uint32_t dst = (b * TC + j) * wcap * ESZ;
+			for (uint32_t sl = (wave - j * S) & (NW - 1); sl < slots; sl += NW) {
+				__builtin_amdgcn_global_load_lds((gptr_t)(src + sl * SLOT * ESZ), (lptr_t)(smem + dst + sl * SLOT * ESZ),
+						16, 0, 0);
+			}
+		}
+	};
+
+	// Windows of pairs with nothing to stage are read (with weight 0): start
+	// from zeros so that no stale bit pattern can be Inf/NaN.
+	for (uint32_t i = tid; i < 2 * TC * wcap * ESZ / 16; i += NT)
+		reinterpret_cast<uint4*>(smem)[i] = make_uint4(0u, 0u, 0u, 0u);
+	__syncthreads();
+
+	PixelStats<CF> st;
+	uint32_t buf = 0;
+	stage(0, 0, 0);
+	for (uint32_t r = 0; r < nrx; ++r) {
+		const float drx = a.delay[static_cast<uint64_t>(r) * npix + p];
+		float ar = 0.0f, ai = 0.0f;
+#pragma clang loop unroll(full)
+		for (int c = 0; c < NTXMAX / TC; ++c) {
+			__builtin_amdgcn_s_waitcnt(0);   // this wave's copies into `buf` landed
+			__syncthreads();                 // everyone's copies landed; `buf ^ 1` is free
+			if (c + 1 < NTXMAX / TC) stage(buf ^ 1u, r, c + 1);
+			else if (r + 1 < nrx) stage(buf ^ 1u, r + 1, 0);
+			const cu2_t pr = prm + r * NTXMAX + c * TC;
+			const cf32_t wr = (cf32_t)(a.w + static_cast<uint64_t>(r) * NTXMAX + c * TC);
+#pragma unroll
+			for (int j = 0; j < TC; ++j) {
+				const uint32_t word = pr[j].word;
+				// byte offset of entry 0 of the window, minus ws entries
+				const int wbase = static_cast<int>((buf * TC + j) * wcap * ESZ);
+				const int base = wbase - static_cast<int>((word & kWsMask) * ESZ);
+				const float pos = dtx[c * TC + j] + drx;
+				const float k = __builtin_amdgcn_fractf(pos);
+				float w = wr[j];
+				int off;
+				if (!(word & kEdgeBit)) {
+					// pos >= lo >= 0: truncation is floor; floor(pos) in
+					// [ws, floor(hi)] by the window construction
+					off = static_cast<int>(pos) * static_cast<int>(ESZ) + base;
+				} else {
+					// record end, nothing staged or padded tx: keep only
+					// 0 <= pos < L-1; excluded lanes read entry 0 with weight 0
+					const bool ok = pos >= 0.0f && pos < lim;
+					w = ok ? w : 0.0f;
+					off = ok ? static_cast<int>(pos) * static_cast<int>(ESZ) + base : wbase;
+				}
+				const E* e = reinterpret_cast<const E*>(smem + off);
+				const float2 v0 = Entry<STORAGE>::get(e[0]);
+				const float2 v1 = Entry<STORAGE>::get(e[1]);
+				ar = fmaf(fmaf(k, v1.x - v0.x, v0.x), w, ar);
+				ai = fmaf(fmaf(k, v1.y - v0.y, v0.y), w, ai);
+			}
+			buf ^= 1u;
+		}
+		st.add_rx(ar, ai, r == 0);
+	}
+	if (valid) st.write(a, p);
+}
+
+template<int STORAGE, int CF, int NTXMAX, int NT>
+void launch_tile_t(const TileArgs& A, uint32_t ntiles, uint32_t tc, hipStream_t s)
+{
+	using E = typename Entry<STORAGE>::T;
+	const size_t lds = 2ull * tc * A.wcap * sizeof(E);
+	auto go = [&](auto kern) {
+		HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+				static_cast<int>(lds)));
+		hipLaunchKernelGGL(kern, dim3(ntiles), dim3(NT), lds, s, A);
+	};
+	if (tc == 16) go(das_tile_kernel<STORAGE, CF, NTXMAX, 16, NT>);
+	else if (tc == 8) go(das_tile_kernel<STORAGE, CF, NTXMAX, 8, NT>);
+	else go(das_tile_kernel<STORAGE, CF, NTXMAX, 4, NT>);
+}
+
+template<int STORAGE, int CF, int NT>
+void launch_tile_n(const TileArgs& A, uint32_t ntiles, uint32_t tc, hipStream_t s)
+{
+	if (A.ntx_pad == 32) launch_tile_t<STORAGE, CF, 32, NT>(A, ntiles, tc, s);
+	else if (A.ntx_pad == 64) launch_tile_t<STORAGE, CF, 64, NT>(A, ntiles, tc, s);
+	else launch_tile_t<STORAGE, CF, 128, NT>(A, ntiles, tc, s);
+}
+
+// Tiled launch for one storage kind (instantiated in das_tile_<storage>.hip).
+template<int STORAGE>
+void launch_tile_storage(int cf, const TileArgs& A, uint32_t ntiles, uint32_t tc, hipStream_t s)
+{
+	switch (cf) {
+	case MISTA_CF_NONE: launch_tile_n<STORAGE, MISTA_CF_NONE, 512>(A, ntiles, tc, s); break;
+	case MISTA_CF_PCF: launch_tile_n<STORAGE, MISTA_CF_PCF, 512>(A, ntiles, tc, s); break;
+	default: launch_tile_n<STORAGE, MISTA_CF_SCF, 512>(A, ntiles, tc, s); break;
+	}
+}
+
+}  // namespace das
+}  // namespace mista

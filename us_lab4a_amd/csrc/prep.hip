@@ -91,6 +91,32 @@ void launch_delay_table(const float* d_grid_xz, uint64_t npix, uint32_t num_elem
 	HIP_CHECK(hipGetLastError());
 }
 
+// 3-D form, Geometry::distance3DZ0 (Geometry.h:77-84): elements at (x_e, y_e, 0)
+// from an explicit position list (rows of the table: active rx elements, then
+// active tx elements), grid points (x, y, z).  FP64 distance, rounded once.
+__global__ void __launch_bounds__(256) delay_table_3d_kernel(const float* __restrict__ grid_xyz, uint64_t npix,
+		const double2* __restrict__ elem_xy, double inv_ct, float* __restrict__ delay)
+{
+	const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint32_t e = blockIdx.y;
+	if (p >= npix) return;
+	const double2 el = elem_xy[e];
+	const double dx = static_cast<double>(grid_xyz[3 * p]) - el.x;
+	const double dy = static_cast<double>(grid_xyz[3 * p + 1]) - el.y;
+	const double z = static_cast<double>(grid_xyz[3 * p + 2]);
+	delay[static_cast<uint64_t>(e) * npix + p] = static_cast<float>(sqrt(dx * dx + dy * dy + z * z) * inv_ct);
+}
+
+void launch_delay_table_3d(const float* d_grid_xyz, uint64_t npix, const double* d_elem_xy, uint32_t rows,
+		double inv_ct, float* d_delay, hipStream_t s)
+{
+	if (npix == 0) return;
+	const dim3 grid(static_cast<uint32_t>((npix + 255) / 256), rows);
+	hipLaunchKernelGGL(delay_table_3d_kernel, grid, dim3(256), 0, s, d_grid_xyz, npix,
+			reinterpret_cast<const double2*>(d_elem_xy), inv_ct, d_delay);
+	HIP_CHECK(hipGetLastError());
+}
+
 // ---------------------------------------------------------------- FIR + DC
 
 constexpr int kFirThreads = 1024;

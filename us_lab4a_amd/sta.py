@@ -228,3 +228,114 @@ class HipSTAProcessor:
         p, d = ctypes.c_float(), ctypes.c_float()
         check(lib().mista_last_timing(self._h, ctypes.byref(p), ctypes.byref(d)), self._h)
         return p.value, d.value
+
+
+@dataclass
+class SA3DConfiguration:
+    """Fields of SA3DConfiguration<float> on the path (SA3DConfiguration.h:37-58):
+    element positions (x, y) per array element and the active tx/rx lists,
+    relative to the base element."""
+    sampling_frequency: float
+    propagation_speed: float
+    center_frequency: float
+    tx_elem_pos: np.ndarray          # [num tx elements][2]
+    rx_elem_pos: np.ndarray          # [num rx elements][2]
+    active_tx_elem: list
+    active_rx_elem: list
+    dead_zone_m: float = 0.0
+
+    @property
+    def num_tx(self):
+        return len(self.active_tx_elem)
+
+    @property
+    def num_rx(self):
+        return len(self.active_rx_elem)
+
+    # the 2-D processor's tensor shape is [num_tx][num_elements][L]
+    @property
+    def num_elements(self):
+        return self.num_rx
+
+    def positions(self, base_element=0):
+        """Positions of the active elements: txElemPos[base + activeTxElem[i]]
+        (Vectorial3DSTAProcessor.h:204-211)."""
+        tx = np.asarray(self.tx_elem_pos, np.float32)[base_element + np.asarray(self.active_tx_elem)]
+        rx = np.asarray(self.rx_elem_pos, np.float32)[base_element + np.asarray(self.active_rx_elem)]
+        return np.ascontiguousarray(tx), np.ascontiguousarray(rx)
+
+
+class HipSTA3DProcessor(HipSTAProcessor):
+    """Vectorial3DSTAProcessor on MI355X (Vectorial3DSTAProcessor.h:50-253):
+    same constructor, prepare(base) and process(grid_data) on points with
+    x, y, z; rx_apod has one entry per active rx element."""
+
+    def __init__(self, config, acquisition, upsampling_factor, coherence_factor, peak_offset, rx_apod,
+                 tx_apod=None, storage="f32", device=0):
+        self.config = config
+        self.acquisition = acquisition
+        self.coherence_factor = coherence_factor if coherence_factor is not None else CoherenceFactor()
+        c = _lib.Sta3dConfig()
+        c.num_tx = config.num_tx
+        c.num_rx = config.num_rx
+        c.upsampling_factor = upsampling_factor
+        c.sampling_frequency = config.sampling_frequency
+        c.propagation_speed = config.propagation_speed
+        c.center_frequency = config.center_frequency
+        c.dead_zone_m = config.dead_zone_m
+        c.peak_offset = peak_offset
+        c.cf_kind = self.coherence_factor.kind
+        c.cf_param = self.coherence_factor.param
+        c.storage = _STORAGE[storage]
+        c.device = device
+        tx, rx = config.positions(0)
+        h = ctypes.c_void_p()
+        check(lib().mista_create_3d(ctypes.byref(c), tx.ctypes.data, rx.ctypes.data, ctypes.byref(h)))
+        self._h = h
+        self._base = 0
+        self.upsampling_factor = upsampling_factor
+        self.storage = storage
+        self._grid_shape = None
+        self._grid_xyz = None
+        self.set_rx_apod(rx_apod)
+        if tx_apod is not None:
+            self.set_tx_apod(tx_apod)
+
+    def prepare(self, base_element):
+        """Vectorial3DSTAProcessor::prepare (:133-138): a new base element moves
+        the active elements."""
+        if self.acquisition is not None:
+            self.acquisition.prepare(base_element)
+        if base_element != self._base:
+            tx, rx = self.config.positions(base_element)
+            check(lib().mista_set_elements_3d(self._h, tx.ctypes.data, rx.ctypes.data), self._h)
+            self._base = base_element
+        check(lib().mista_prepare(self._h, base_element), self._h)
+
+    def set_grid(self, grid, ny=1):
+        """grid: structured XYZValueFactor array [n1][n2] or float32 [n1][n2][3]
+        (x, y, z); ny > 1 declares a volume with n1 = nx * ny (y fastest)."""
+        if grid.dtype.names:
+            xyz = np.ascontiguousarray(np.stack([grid["x"], grid["y"], grid["z"]], axis=-1), dtype=np.float32)
+        else:
+            xyz = np.ascontiguousarray(grid, dtype=np.float32)
+        n1, n2 = xyz.shape[:2]
+        check(lib().mista_set_grid_xyz(self._h, xyz.ctypes.data, n1, n2, ny), self._h)
+        self._grid_shape = (n1, n2)
+        self._grid_xyz = xyz
+
+    def process(self, grid_data):
+        xyz = np.ascontiguousarray(np.stack([grid_data["x"], grid_data["y"], grid_data["z"]], axis=-1), np.float32)
+        if self._grid_shape != xyz.shape[:2] or not np.array_equal(xyz, self._grid_xyz):
+            self.set_grid(xyz)
+        cfg = self.config
+        echoes = np.stack([self.acquisition.execute(t) for t in cfg.active_tx_elem]).astype(np.float32)
+        value, factor = self.process_echoes(echoes)
+        grid_data["value"] = value
+        grid_data["factor"] = factor
+
+    def download_delays(self):
+        n1, n2 = self._grid_shape
+        out = np.empty((self.config.num_rx + self.config.num_tx, n1 * n2), np.float32)
+        check(lib().mista_download_delays(self._h, out.ctypes.data), self._h)
+        return out

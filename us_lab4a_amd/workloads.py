@@ -145,7 +145,121 @@ def tiny(num_elements=8, num_samples=256, upsampling=4, cf="none", n_x=16, n_z=1
                       snr_db=30.0)
 
 
-WORKLOADS = {"sta2d_32_small": sta2d_32_small, "sta2d_128_pcf": sta2d_128_pcf}
+@dataclass
+class Workload3D:
+    """Sparse 3-D STA on a 2-D matrix array (Vectorial3DSTAProcessor): all rx
+    elements active, a seeded subset of tx elements; volume grid laid out as
+    n1 = nx * ny columns (y fastest) by n2 = nz rows."""
+    name: str
+    array_nx: int
+    array_ny: int
+    num_tx: int
+    num_samples: int
+    upsampling: int
+    pitch: float = 0.3 * MM
+    fs: float = 40e6
+    c: float = 1540.0
+    fc: float = 3e6
+    bw: float = 0.6
+    snr_db: float = 35.0
+    cf: str = "sign_coherence_factor"
+    cf_param: float = 1.0
+    extent_xy: tuple = (-10 * MM, 10 * MM)
+    extent_z: tuple = (10 * MM, 70 * MM)
+    n_x: int = 128
+    n_y: int = 128
+    n_z: int = 256
+    points: int = 500
+    seed_tx: int = 21
+    seed_points: int = 22
+    seed_noise: int = 23
+    peak_offset: float = 0.0
+    dead_zone_m: float = 0.0
+
+    @property
+    def num_rx(self):
+        return self.array_nx * self.array_ny
+
+    @property
+    def pixel_pairs(self):
+        return self.n_x * self.n_y * self.n_z * self.num_tx * self.num_rx
+
+    def element_xy(self):
+        return imaging.element_positions_2d_array(self.pitch, self.array_nx, 0.0, self.pitch, self.array_ny, 0.0)
+
+    def active_tx(self):
+        """Seeded choice of num_tx distinct elements, in ascending order."""
+        u = synth.uniform01(self.seed_tx, np.arange(self.num_rx))
+        return sorted(int(i) for i in np.argsort(u, kind="stable")[:self.num_tx])
+
+    def sa3d_config(self):
+        from .sta import SA3DConfiguration
+        pos = self.element_xy()
+        return SA3DConfiguration(sampling_frequency=self.fs, propagation_speed=self.c, center_frequency=self.fc,
+                                 tx_elem_pos=pos, rx_elem_pos=pos, active_tx_elem=self.active_tx(),
+                                 active_rx_elem=list(range(self.num_rx)), dead_zone_m=self.dead_zone_m)
+
+    def coherence(self):
+        from .sta import CoherenceFactor
+        return CoherenceFactor(self.cf, self.cf_param)
+
+    def rx_apodization(self):
+        return np.ones(self.num_rx, np.float32)
+
+    def grid_xyz(self):
+        """[nx*ny][nz][3] float32 (x, y, z), spacing as fillSequenceFromStartToEndWithSize."""
+        xs = imaging.fill_sequence_start_end_size(self.extent_xy[0], self.extent_xy[1], self.n_x)
+        ys = imaging.fill_sequence_start_end_size(self.extent_xy[0], self.extent_xy[1], self.n_y)
+        zs = imaging.fill_sequence_start_end_size(self.extent_z[0], self.extent_z[1], self.n_z)
+        g = np.empty((self.n_x, self.n_y, self.n_z, 3), np.float32)
+        g[..., 0] = np.asarray(xs, np.float32)[:, None, None]
+        g[..., 1] = np.asarray(ys, np.float32)[None, :, None]
+        g[..., 2] = np.asarray(zs, np.float32)[None, None, :]
+        return g.reshape(self.n_x * self.n_y, self.n_z, 3)
+
+    def scatterers(self):
+        return synth.point_scatterers(self.seed_points, self.points, self.extent_xy, self.extent_z,
+                                      y_range=self.extent_xy)
+
+    def positions_xyz(self):
+        pos = self.element_xy()
+        xyz = np.zeros((self.num_rx, 3), np.float32)
+        xyz[:, :2] = pos
+        return xyz[self.active_tx()], xyz
+
+    def echoes_cpu(self):
+        tx, rx = self.positions_xyz()
+        e = synth.echoes_cpu(tx, rx, self.scatterers(), self.num_samples, self.fs, self.c, self.fc, self.bw)
+        if self.snr_db is not None:
+            e, _ = synth.add_noise_cpu(e, self.snr_db, self.seed_noise)
+        return e
+
+    def echoes_gpu(self, d_out, stream=None):
+        tx, rx = self.positions_xyz()
+        return synth.echoes_gpu(tx, rx, self.scatterers(), self.num_samples, self.fs, self.c, self.fc, self.bw,
+                                d_out, snr_db=self.snr_db, seed=self.seed_noise, stream=stream)
+
+    def oracle_cfg(self):
+        import oracle  # test infrastructure
+        cf = {"none": oracle.CF_NONE, "phase_coherence_factor": oracle.CF_PCF,
+              "sign_coherence_factor": oracle.CF_SCF}[self.cf]
+        return oracle.make_cfg3d(self.num_tx, self.num_rx, self.num_samples, self.upsampling, self.fs, self.c,
+                                 self.fc, dead_zone_m=self.dead_zone_m, peak_offset=self.peak_offset, cf_kind=cf,
+                                 cf_param=self.cf_param)
+
+
+def sta3d_sparse():
+    return Workload3D(name="sta3d_sparse", array_nx=32, array_ny=32, num_tx=64, num_samples=4096, upsampling=4)
+
+
+def sta3d_small(cf="sign_coherence_factor", n=(12, 10, 24)):
+    """8 x 8 array, 16 tx, 1024 samples: small enough for the oracle."""
+    return Workload3D(name="sta3d_small", array_nx=8, array_ny=8, num_tx=16, num_samples=1024, upsampling=4,
+                      cf=cf, extent_xy=(-3 * MM, 3 * MM), extent_z=(5 * MM, 17 * MM), n_x=n[0], n_y=n[1], n_z=n[2],
+                      points=6)
+
+
+WORKLOADS = {"sta2d_32_small": sta2d_32_small, "sta2d_128_pcf": sta2d_128_pcf, "sta3d_sparse": sta3d_sparse}
 
 
 def db_error(env, env_ref):
