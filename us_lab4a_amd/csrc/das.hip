@@ -222,19 +222,19 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 	}
 	a.inv_num = 1.0f / static_cast<float>(static_cast<uint64_t>(g.ntx) * g.nrx);
 
-	// Tile kernel.  p0: tile shape (below); p1: pairs per staged chunk (4/8/16).
+	// Tile kernel.  p0: tile shape, p1: chunking (below).
 	// window start entries are u32 (channel * L + ws): tensor below 2^32 entries
 	bool tiled = (t.variant != 1) && g.ntx_pad <= 128 && g.L < (1u << 21) - 64 &&
 			static_cast<uint64_t>(g.ntx) * g.nrx * g.L < (1ull << 32) - (1ull << 22);
 	if (tiled) {
-		// 512 lanes per tile.  Planes: 32x16 (x, z); volumes: 8x8x8 (x, y, z).
-		// p0 overrides: 2 = 16x32, 3 = 64x8, 4 = 8x8x8.
+		// 256 lanes per tile.  Planes: 16x16 (x, z); volumes: 8x8x4 (x, y, z).
+		// p0 overrides: 2 = 8x32, 3 = 32x8, 4 = 8x8x4.
 		const uint32_t ny = g.ny ? g.ny : 1u;
-		uint32_t tw1 = 32, tw2 = 1, tzh = 16;
-		if (ny > 1) { tw1 = 8; tw2 = 8; tzh = 8; }
-		if (t.p0 == 2) { tw1 = 16; tw2 = 1; tzh = 32; }
-		else if (t.p0 == 3) { tw1 = 64; tw2 = 1; tzh = 8; }
-		else if (t.p0 == 4) { tw1 = 8; tw2 = 8; tzh = 8; }
+		uint32_t tw1 = 16, tw2 = 1, tzh = 16;
+		if (ny > 1) { tw1 = 8; tw2 = 8; tzh = 4; }
+		if (t.p0 == 2) { tw1 = 8; tw2 = 1; tzh = 32; }
+		else if (t.p0 == 3) { tw1 = 32; tw2 = 1; tzh = 8; }
+		else if (t.p0 == 4) { tw1 = 8; tw2 = 8; tzh = 4; }
 		const uint32_t esz = static_cast<uint32_t>(storage_bytes(g.storage));
 		const uint32_t slot = 64 * (16 / esz);
 		const uint32_t nx = g.n1 / ny;
@@ -269,12 +269,31 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 			scratch.params_key = key;
 			scratch.params_offset = g.offset;
 		}
-		// pairs per staged chunk: the largest of 16/8/4 (or p1) whose two
-		// buffers fit the LDS budget
-		uint32_t tc = (t.p1 == 4 || t.p1 == 8 || t.p1 == 16) ? static_cast<uint32_t>(t.p1) : 16u;
-		const size_t budget = 150 * 1024;
-		while (tc > 4 && 2ull * tc * scratch.wcap * esz > budget) tc /= 2;
-		if (scratch.max_slots > 511 || 2ull * tc * scratch.wcap * esz > budget) tiled = false;
+		// Pairs per chunk (tc, 4 or 8) and chunk buffers in the LDS ring
+		// (nbuf): the most pairs in flight, (nbuf - 1) * tc, that fit the LDS
+		// budget and the 6-bit vmcnt of the per-chunk waits.  p1 overrides:
+		// byte 0 tc, byte 1 nbuf, byte 2 LDS budget in KiB (default 76: two
+		// workgroups share a CU, which measures faster than one workgroup
+		// with a deeper ring in 150 KiB).
+		const uint32_t want_kb = (static_cast<uint32_t>(t.p1) >> 16) & 0xFF;
+		const size_t budget = static_cast<size_t>(want_kb ? std::min(want_kb, 158u) : 76u) * 1024;
+		const uint32_t S = scratch.wcap / slot;
+		auto fit = [&](uint32_t tc, uint32_t want) -> uint32_t {
+			const size_t bb = static_cast<size_t>(tc) * scratch.wcap * esz + kTileThreads * 4;
+			uint32_t nb = static_cast<uint32_t>(std::min<size_t>(16, budget / bb));
+			if (want) nb = std::min(nb, want);
+			const uint32_t per = tc * S / (kTileThreads / 64) + 1;   // copies per wave per chunk, at most
+			while (nb > 2 && (nb - 2) * per > 63) --nb;
+			return (nb >= 2 && 2 * per <= 63) ? nb : 0u;
+		};
+		const uint32_t want_tc = static_cast<uint32_t>(t.p1) & 0xFF, want_nb = (static_cast<uint32_t>(t.p1) >> 8) & 0xFF;
+		uint32_t tc = 0, nbuf = 0;
+		for (uint32_t c : {8u, 4u}) {
+			if (want_tc && c != want_tc) continue;
+			const uint32_t nb = fit(c, want_nb);
+			if (nb && (nb - 1) * c > (nbuf ? (nbuf - 1) * tc : 0u)) { tc = c; nbuf = nb; }
+		}
+		if (scratch.max_slots > 511 || nbuf == 0) tiled = false;
 		if (tiled) {
 			TileArgs A;
 			A.d = a;
@@ -282,6 +301,7 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 			A.params = scratch.params.as<const WinParam>();
 			A.wcap = scratch.wcap;
 			A.ntx_pad = g.ntx_pad;
+			A.nbuf = nbuf;
 			switch (g.storage) {
 			case MISTA_STORAGE_F32: launch_tile_storage<MISTA_STORAGE_F32>(g.cf_kind, A, ntiles, tc, s); break;
 			case MISTA_STORAGE_F16: launch_tile_storage<MISTA_STORAGE_F16>(g.cf_kind, A, ntiles, tc, s); break;

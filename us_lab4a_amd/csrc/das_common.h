@@ -164,25 +164,76 @@ struct TileArgs {
 	const WinParam* params;   // [ntiles][nrx][ntx_pad]
 	uint32_t wcap;            // LDS window capacity per pair (entries, multiple of SLOT)
 	uint32_t ntx_pad;         // transmits padded to the instantiated count
+	uint32_t nbuf;            // chunk buffers in the LDS ring (>= 2)
 };
 
-template<int STORAGE, int CF, int NTXMAX, int TC, int NT>
-__global__ void __launch_bounds__(NT) das_tile_kernel(const TileArgs A)
+constexpr uint32_t kTileThreads = 256;
+
+// Direct global -> LDS copies (LDS DMA).  Each lane moves 16 (4) bytes from
+// its own address to lds + 16 (4) * lane; lds must be wave-uniform.  Written
+// as asm so that the compiler does not treat the asynchronous LDS write as
+// aliasing every later LDS read (it would wait for all copies in flight);
+// the kernel waits for them itself with s_waitcnt vmcnt.
+// m0 is declared clobbered (it is reserved, hence the silenced warning); the
+// kernels using these never rely on m0 themselves.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void lds_dma16(const void* g, uint32_t lds)
+{
+	asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" :: "s"(lds), "v"(g) : "memory", "m0");
+}
+__device__ __forceinline__ void lds_dma4(const void* g, uint32_t lds)
+{
+	asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off" :: "s"(lds), "v"(g) : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+// Wait until at most n (wave-uniform) vector memory operations of this wave
+// are outstanding, rounding n down to a power of two: waits at least as long
+// as s_waitcnt vmcnt(n), with 7 immediates instead of 64.  The expcnt and
+// lgkmcnt fields stay at their maxima (no wait on them).
+#define MISTA_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
+__device__ __forceinline__ void wait_vmcnt_at_most(uint32_t n)
+{
+	if (n >= 32) MISTA_VMCNT(32);
+	else if (n >= 16) MISTA_VMCNT(16);
+	else if (n >= 8) MISTA_VMCNT(8);
+	else if (n >= 4) MISTA_VMCNT(4);
+	else if (n >= 2) MISTA_VMCNT(2);
+	else if (n >= 1) MISTA_VMCNT(1);
+	else MISTA_VMCNT(0);
+}
+
+// Chunk i of a tile = pairs (rx r, tx c*TC .. c*TC+TC-1), r = i / CPR,
+// c = i % CPR.  Chunks stream through a ring of nbuf LDS buffers, each
+// holding TC windows of wcap entries plus the tile's rx delays (one float per
+// lane, copied with the first chunk of every rx).  Chunk i + nbuf - 1 is
+// copied while chunk i is computed, so nbuf - 1 chunks of copies are in
+// flight.  Every wave issues the same number of copies per chunk (windows
+// narrower than the widest re-copy their first slot into unused space), so
+// the number of copies issued after chunk i is known and the wave waits for
+// chunk i alone with s_waitcnt vmcnt.
+template<int STORAGE, int CF, int NTXMAX, int TC>
+__global__ void __launch_bounds__(kTileThreads)
+das_tile_kernel(const TileArgs A)
 {
 	using E = typename Entry<STORAGE>::T;
+	constexpr uint32_t NT = kTileThreads;
 	constexpr uint32_t ESZ = sizeof(E);
 	constexpr uint32_t EPL = 16 / ESZ;      // entries per 16-byte lane load
 	constexpr uint32_t SLOT = 64 * EPL;     // entries per wave load instruction
 	constexpr uint32_t NW = NT / 64;
-	extern __shared__ __attribute__((aligned(16))) unsigned char smem[];   // [2][TC][wcap] entries
+	constexpr uint32_t CPR = NTXMAX / TC;   // chunks per rx
+	extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
 
 	const DasArgs& a = A.d;
 	// XCD-aware order: workgroups are dealt round-robin to the 8 XCDs
 	// (block b runs on XCD b % 8), so give XCD x one contiguous run of tiles;
 	// tiles running at the same time on one XCD then share windows in its L2.
-	const uint32_t ntiles = gridDim.x, q = ntiles / 8, rm = ntiles % 8;
-	const uint32_t xcd = blockIdx.x % 8, tile = xcd * q + min(xcd, rm) + blockIdx.x / 8;
-	const uint32_t tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+	const uint32_t ntiles = gridDim.x, q8 = ntiles / 8, rm8 = ntiles % 8;
+	const uint32_t xcd = blockIdx.x % 8, tile = xcd * q8 + min(xcd, rm8) + blockIdx.x / 8;
+	const uint32_t tid = threadIdx.x, lane = tid % 64;
+	const uint32_t wave = __builtin_amdgcn_readfirstlane(tid / 64);   // uniform: scalar copy loops
 	uint32_t col, row;
 	bool valid;
 	A.tg.pixel(tile, tid, col, row, valid);
@@ -190,113 +241,135 @@ __global__ void __launch_bounds__(NT) das_tile_kernel(const TileArgs A)
 	const uint64_t p = static_cast<uint64_t>(col) * A.tg.n2 + row;
 	const uint32_t L = a.L, ntx = a.ntx, nrx = a.nrx;
 	const float lim = static_cast<float>(L - 1u);
-	const uint32_t wcap = A.wcap;
+	const uint32_t wcap = A.wcap, NB = A.nbuf;
+	const uint32_t S = wcap / SLOT;                   // slots of the widest window
+	const uint32_t QW = TC * S / NW;                  // window copies per wave per chunk
+	const uint32_t win_bytes = TC * wcap * ESZ;
+	const uint32_t buf_bytes = win_bytes + NT * 4;    // windows + rx delays
 	const cu2_t prm = (cu2_t)(A.params + static_cast<uint64_t>(tile) * nrx * NTXMAX);
 	const unsigned char* tensor = static_cast<const unsigned char*>(a.tensor);
+	const uint32_t lds0 = static_cast<uint32_t>(reinterpret_cast<size_t>((__attribute__((address_space(3))) void*)smem));
 
-	// tx delays of this lane's pixel, offset folded in: (offset + d_tx)
+	// tx delays of this lane's point, offset folded in: (offset + d_tx).
 	// Transmits are padded to NTXMAX: padded ones have weight 0, an empty
 	// window and a position far below 0, so they take the excluded path.
 	float dtx[NTXMAX];
 #pragma unroll
-	for (int t = 0; t < NTXMAX; ++t)
-		dtx[t] = (static_cast<uint32_t>(t) < ntx) ? a.offset + a.delay[static_cast<uint64_t>(a.first_tx + t) * npix + p] : -1.0e30f;
+	for (int t = 0; t < NTXMAX; ++t) {
+		const uint32_t tt = min(static_cast<uint32_t>(t), ntx - 1);   // unconditional load, no branch
+		const float d = a.delay[static_cast<uint64_t>(a.first_tx + tt) * npix + p];
+		dtx[t] = (static_cast<uint32_t>(t) < ntx) ? a.offset + d : -1.0e30f;
+	}
 
-	// Copy the windows of pairs (r, c*TC .. c*TC+TC-1) into buffer b: wave w
-	// issues load slots w, w + NW, ... of every pair; each lane moves 16 bytes.
-	// Slot sl of pair j is load number j * S + sl of the chunk (S = slots of
-	// the widest window) and goes to wave (j * S + sl) % NW, so the loads of a
-	// chunk spread over all waves.
-	const uint32_t S = wcap / SLOT;
-	auto stage = [&](uint32_t b, uint32_t r, uint32_t c) {
+	// Issue the copies of chunk i into buffer b.  Slot sl of pair j is copy
+	// number j * S + sl of the chunk and goes to wave (j * S + sl) % NW.
+	const uint32_t nchunks = nrx * CPR;
+	auto issue = [&](uint32_t i, uint32_t b) {
+		const uint32_t r = i / CPR, c = i % CPR;
 		const cu2_t pr = prm + r * NTXMAX + c * TC;
+		const uint32_t bbase = lds0 + b * buf_bytes;
 #pragma unroll
 		for (uint32_t j = 0; j < TC; ++j) {
 			const uint32_t slots = (pr[j].word >> 21) & 511u;
 			const unsigned char* src = tensor + static_cast<uint64_t>(pr[j].start) * ESZ + lane * 16;
-			const uint32_t dst = (b * TC + j) * wcap * ESZ;
-			for (uint32_t sl = (wave - j * S) & (NW - 1); sl < slots; sl += NW) {
-				__builtin_amdgcn_global_load_lds((gptr_t)(src + sl * SLOT * ESZ), (lptr_t)(smem + dst + sl * SLOT * ESZ),
-						16, 0, 0);
+			const uint32_t dst = bbase + j * wcap * ESZ;
+			for (uint32_t sl = (wave - j * S) & (NW - 1); sl < S; sl += NW) {
+				const uint32_t from = sl < slots ? sl : 0u;
+				lds_dma16(src + from * SLOT * ESZ, __builtin_amdgcn_readfirstlane(dst + sl * SLOT * ESZ));
 			}
 		}
+		if (c == 0)
+			lds_dma4(a.delay + static_cast<uint64_t>(r) * npix + p, __builtin_amdgcn_readfirstlane(bbase + win_bytes + wave * 256));
 	};
+	// copies issued by this wave for chunk k
+	auto copies = [&](uint32_t k) { return QW + ((k % CPR) == 0 ? 1u : 0u); };
 
 	// Windows of pairs with nothing to stage are read (with weight 0): start
 	// from zeros so that no stale bit pattern can be Inf/NaN.
-	for (uint32_t i = tid; i < 2 * TC * wcap * ESZ / 16; i += NT)
+	for (uint32_t i = tid; i < NB * buf_bytes / 16; i += NT)
 		reinterpret_cast<uint4*>(smem)[i] = make_uint4(0u, 0u, 0u, 0u);
 	__syncthreads();
 
+	const uint32_t pre = min(NB - 1, nchunks);
+	for (uint32_t k = 0; k < pre; ++k) issue(k, k);
+
 	PixelStats<CF> st;
-	uint32_t buf = 0;
-	stage(0, 0, 0);
+	uint32_t i = 0, b = 0;
 	for (uint32_t r = 0; r < nrx; ++r) {
-		const float drx = a.delay[static_cast<uint64_t>(r) * npix + p];
-		float ar = 0.0f, ai = 0.0f;
+		float drx = 0.0f, ar = 0.0f, ai = 0.0f;
 #pragma clang loop unroll(full)
-		for (int c = 0; c < NTXMAX / TC; ++c) {
-			__builtin_amdgcn_s_waitcnt(0);   // this wave's copies into `buf` landed
-			__syncthreads();                 // everyone's copies landed; `buf ^ 1` is free
-			if (c + 1 < NTXMAX / TC) stage(buf ^ 1u, r, c + 1);
-			else if (r + 1 < nrx) stage(buf ^ 1u, r + 1, 0);
+		for (int c = 0; c < static_cast<int>(CPR); ++c, ++i) {
+			// copies issued after chunk i: chunks i+1 .. min(i+NB-2, last)
+			uint32_t after = 0;
+			const uint32_t hi = min(i + NB - 2, nchunks - 1);
+			for (uint32_t k = i + 1; k <= hi; ++k) after += copies(k);
+			wait_vmcnt_at_most(after);         // this wave's copies of chunk i landed
+			__syncthreads();                   // everyone's did; buffer of chunk i-1 is free
+			if (i + NB - 1 < nchunks) issue(i + NB - 1, (b + NB - 1) % NB);
+			const uint32_t bbase = b * buf_bytes;
+			if (c == 0) drx = *reinterpret_cast<const float*>(smem + bbase + win_bytes + tid * 4);
+			// Register fences: this chunk's positions are computed after the
+			// barrier, and its accumulation is finished before the next one,
+			// so only one chunk's temporaries are ever live.
+			asm volatile("" : "+v"(drx));
 			const cu2_t pr = prm + r * NTXMAX + c * TC;
 			const cf32_t wr = (cf32_t)(a.w + static_cast<uint64_t>(r) * NTXMAX + c * TC);
 #pragma unroll
 			for (int j = 0; j < TC; ++j) {
-				const uint32_t word = pr[j].word;
-				// byte offset of entry 0 of the window, minus ws entries
-				const int wbase = static_cast<int>((buf * TC + j) * wcap * ESZ);
-				const int base = wbase - static_cast<int>((word & kWsMask) * ESZ);
+				// One form for every pair: lanes outside 0 <= pos < L-1 get
+				// weight 0; the entry index is clamped into the window, which
+				// changes nothing for in-range lanes (floor(pos) - ws lies in
+				// [0, need - 2] by the window construction) and keeps excluded
+				// lanes' reads inside the buffer.
+				const int ws = static_cast<int>(pr[j].word & kWsMask);
 				const float pos = dtx[c * TC + j] + drx;
 				const float k = __builtin_amdgcn_fractf(pos);
-				float w = wr[j];
-				int off;
-				if (!(word & kEdgeBit)) {
-					// pos >= lo >= 0: truncation is floor; floor(pos) in
-					// [ws, floor(hi)] by the window construction
-					off = static_cast<int>(pos) * static_cast<int>(ESZ) + base;
-				} else {
-					// record end, nothing staged or padded tx: keep only
-					// 0 <= pos < L-1; excluded lanes read entry 0 with weight 0
-					const bool ok = pos >= 0.0f && pos < lim;
-					w = ok ? w : 0.0f;
-					off = ok ? static_cast<int>(pos) * static_cast<int>(ESZ) + base : wbase;
-				}
+				const bool ok = pos >= 0.0f && pos < lim;
+				const float w = ok ? wr[j] : 0.0f;
+				const float pc = __builtin_amdgcn_fmed3f(pos, 0.0f, lim);   // NaN-free, no int overflow
+				const int rel = min(max(static_cast<int>(pc) - ws, 0), static_cast<int>(wcap) - 2);
+				const int off = rel * static_cast<int>(ESZ) + static_cast<int>(bbase + j * wcap * ESZ);
 				const E* e = reinterpret_cast<const E*>(smem + off);
 				const float2 v0 = Entry<STORAGE>::get(e[0]);
 				const float2 v1 = Entry<STORAGE>::get(e[1]);
 				ar = fmaf(fmaf(k, v1.x - v0.x, v0.x), w, ar);
 				ai = fmaf(fmaf(k, v1.y - v0.y, v0.y), w, ai);
 			}
-			buf ^= 1u;
+			asm volatile("" : "+v"(ar), "+v"(ai));
+			b = (b + 1 == NB) ? 0u : b + 1;
 		}
 		st.add_rx(ar, ai, r == 0);
 	}
+	MISTA_VMCNT(0);
 	if (valid) st.write(a, p);
 }
 
-template<int STORAGE, int CF, int NTXMAX, int NT>
+// LDS bytes of one chunk buffer: TC windows of wcap entries + the rx delays.
+template<int STORAGE>
+inline size_t tile_buffer_bytes(uint32_t tc, uint32_t wcap)
+{
+	return static_cast<size_t>(tc) * wcap * sizeof(typename Entry<STORAGE>::T) + kTileThreads * 4;
+}
+
+template<int STORAGE, int CF, int NTXMAX>
 void launch_tile_t(const TileArgs& A, uint32_t ntiles, uint32_t tc, hipStream_t s)
 {
-	using E = typename Entry<STORAGE>::T;
-	const size_t lds = 2ull * tc * A.wcap * sizeof(E);
+	const size_t lds = A.nbuf * tile_buffer_bytes<STORAGE>(tc, A.wcap);
 	auto go = [&](auto kern) {
 		HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
 				static_cast<int>(lds)));
-		hipLaunchKernelGGL(kern, dim3(ntiles), dim3(NT), lds, s, A);
+		hipLaunchKernelGGL(kern, dim3(ntiles), dim3(kTileThreads), lds, s, A);
 	};
-	if (tc == 16) go(das_tile_kernel<STORAGE, CF, NTXMAX, 16, NT>);
-	else if (tc == 8) go(das_tile_kernel<STORAGE, CF, NTXMAX, 8, NT>);
-	else go(das_tile_kernel<STORAGE, CF, NTXMAX, 4, NT>);
+	if (tc == 8) go(das_tile_kernel<STORAGE, CF, NTXMAX, 8>);
+	else go(das_tile_kernel<STORAGE, CF, NTXMAX, 4>);
 }
 
-template<int STORAGE, int CF, int NT>
+template<int STORAGE, int CF>
 void launch_tile_n(const TileArgs& A, uint32_t ntiles, uint32_t tc, hipStream_t s)
 {
-	if (A.ntx_pad == 32) launch_tile_t<STORAGE, CF, 32, NT>(A, ntiles, tc, s);
-	else if (A.ntx_pad == 64) launch_tile_t<STORAGE, CF, 64, NT>(A, ntiles, tc, s);
-	else launch_tile_t<STORAGE, CF, 128, NT>(A, ntiles, tc, s);
+	if (A.ntx_pad == 32) launch_tile_t<STORAGE, CF, 32>(A, ntiles, tc, s);
+	else if (A.ntx_pad == 64) launch_tile_t<STORAGE, CF, 64>(A, ntiles, tc, s);
+	else launch_tile_t<STORAGE, CF, 128>(A, ntiles, tc, s);
 }
 
 // Tiled launch for one storage kind (instantiated in das_tile_<storage>.hip).
@@ -304,9 +377,9 @@ template<int STORAGE>
 void launch_tile_storage(int cf, const TileArgs& A, uint32_t ntiles, uint32_t tc, hipStream_t s)
 {
 	switch (cf) {
-	case MISTA_CF_NONE: launch_tile_n<STORAGE, MISTA_CF_NONE, 512>(A, ntiles, tc, s); break;
-	case MISTA_CF_PCF: launch_tile_n<STORAGE, MISTA_CF_PCF, 512>(A, ntiles, tc, s); break;
-	default: launch_tile_n<STORAGE, MISTA_CF_SCF, 512>(A, ntiles, tc, s); break;
+	case MISTA_CF_NONE: launch_tile_n<STORAGE, MISTA_CF_NONE>(A, ntiles, tc, s); break;
+	case MISTA_CF_PCF: launch_tile_n<STORAGE, MISTA_CF_PCF>(A, ntiles, tc, s); break;
+	default: launch_tile_n<STORAGE, MISTA_CF_SCF>(A, ntiles, tc, s); break;
 	}
 }
 
