@@ -39,6 +39,12 @@ def parse():
     ap.add_argument("--kernel", type=int, default=0, help="DAS variant (0 auto, 1 gather, 2 tiled)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--mode", default="frame", choices=["frame", "batch", "serve"],
+                    help="frame: the headline line; batch: FP16 frames through the serving engine vs plain "
+                         "FP32 frames; serve: Poisson arrivals of a mixed-shape job stream")
+    ap.add_argument("--inflight", type=int, default=4)
+    ap.add_argument("--rates", default="50,200,1000", help="serve mode: arrival rates (frames/s)")
+    ap.add_argument("--jobs", type=int, default=48, help="serve mode: jobs per rate")
     return ap.parse_args()
 
 
@@ -91,11 +97,145 @@ def cpu_baseline(wl, threads):
     }
 
 
+def batch_mode(args):
+    """FP16 storage + engine batching (several frames in flight, overlapping
+    prep and DAS) against the plain FP32 path one frame at a time, same
+    frame, same echoes."""
+    import torch
+    from us_lab4a_amd import workloads
+    from us_lab4a_amd.serve import Engine, Job2D
+    from us_lab4a_amd.sta import CoherenceFactor, HipSTAProcessor
+    wl = workloads.WORKLOADS[args.workload]()
+    npix = wl.n_x * wl.n_z
+    dev = torch.device("cuda:0")
+    d_echoes = torch.empty((wl.num_tx, wl.num_elements, wl.num_samples), dtype=torch.float32, device=dev)
+    wl.echoes_gpu(d_echoes)
+    torch.cuda.synchronize()
+    # plain FP32, one frame at a time
+    proc = HipSTAProcessor(wl.sta_config(), None, wl.upsampling, CoherenceFactor(wl.cf, wl.cf_param),
+                           wl.peak_offset, wl.rx_apodization(), tx_apod=wl.tx_apodization(), storage="f32")
+    proc.set_grid(wl.grid())
+    v32 = torch.empty(npix, device=dev)
+    f32 = torch.empty(npix, device=dev)
+    for _ in range(args.warmup):
+        proc.process_device(d_echoes, wl.num_samples, v32, f32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        proc.process_device(d_echoes, wl.num_samples, v32, f32)
+    torch.cuda.synchronize()
+    fps32 = args.steps / (time.perf_counter() - t0)
+    proc.close()
+    # FP16 storage, frames through the engine, args.inflight in flight
+    outs = [(torch.empty(npix, device=dev), torch.empty(npix, device=dev)) for _ in range(args.inflight)]
+
+    def job(k):
+        v, f = outs[k % len(outs)]
+        return Job2D(config=wl.sta_config(), upsampling=wl.upsampling, coherence=CoherenceFactor(wl.cf, wl.cf_param),
+                     grid=wl.grid(), rx_apod=wl.rx_apodization(), tx_apod=wl.tx_apodization(), echoes=d_echoes,
+                     value=v, factor=f, storage="f16", peak_offset=wl.peak_offset)
+    with Engine(max_inflight=args.inflight, max_queued=4 * args.steps) as eng:
+        for t in [eng.submit(job(k)) for k in range(max(args.warmup, args.inflight))]:
+            assert eng.wait(t)["ok"]
+        t0 = time.perf_counter()
+        tickets = [eng.submit(job(k)) for k in range(args.steps)]
+        lat = [eng.wait(t)["latency_ms"] for t in tickets]
+        fps16 = args.steps / (time.perf_counter() - t0)
+    torch.cuda.synchronize()
+    v16 = outs[(args.steps - 1) % len(outs)][0]
+    ref = v32.double()
+    err_db = 20 * math.log10(float((v16.double() - ref).abs().max() / ref.abs().max()))
+    out = {"metric": "frames/s: FP16 storage + batched engine vs plain FP32", "value": round(fps16, 3),
+           "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "fp32_plain_frames_per_s": round(fps32, 3),
+           "speedup_vs_fp32_plain": round(fps16 / fps32, 3), "inflight": args.inflight,
+           "envelope_error_db_vs_fp32_gpu": round(err_db, 2),
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                          "p99": round(float(np.percentile(lat, 99)), 3)},
+           "config": {"workload": wl.name, "storage": "f16"}, "data": "synthetic"}
+    print(json.dumps(out), flush=True)
+
+
+def serve_mode(args):
+    """Poisson arrivals of 2-D jobs with seeded uneven shapes (elements in
+    {32,64,128,192,256}, samples in {2048,4096,8192}, grids 128x128 ..
+    512x1024, coherence none/PCF/SCF), FP16 storage, U = 8: p50/p99 latency
+    and aggregate Gpixel*pair/s per arrival rate."""
+    import torch
+    from us_lab4a_amd import workloads, _lib
+    from us_lab4a_amd.serve import Engine, Job2D
+    from us_lab4a_amd.sta import CoherenceFactor
+    MM = 1e-3
+    rng = np.random.default_rng(5)
+    cfs = ["none", "phase_coherence_factor", "sign_coherence_factor"]
+    grids = [(128, 128), (256, 256), (512, 512), (512, 1024)]
+    catalog = []
+    for i in range(8):   # a seeded catalog of shapes; jobs draw from it
+        N = int(rng.choice([32, 64, 128, 192, 256]))
+        ns = int(rng.choice([2048, 4096, 8192]))
+        gx, gz = grids[int(rng.integers(len(grids)))]
+        half = N * 0.3 * MM / 2
+        zmax = min(65 * MM, ns / 40e6 * 1540 / 2 * 0.9)
+        wl = workloads.Workload2D(name=f"serve{i}", num_elements=N, num_samples=ns, upsampling=8, cf=cfs[i % 3],
+                                  grid_x=(-half, half), grid_z=(5 * MM, zmax), n_x=gx, n_z=gz, points=50,
+                                  point_x=(-half, half), point_z=(5 * MM, zmax), speckle_density=2000.0,
+                                  seed_points=300 + i, seed_speckle=400 + i, seed_noise=500 + i)
+        d = torch.empty((wl.num_tx, N, ns), dtype=torch.float32, device="cuda")
+        wl.echoes_gpu(d)
+        catalog.append((wl, d))
+        print(f"[bench] shape {i}: {N} el, {ns} samples, grid {gx}x{gz}, {wl.cf}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    results = []
+    with Engine(max_inflight=args.inflight, max_queued=1024) as eng:
+        def job(k):
+            wl, d = catalog[k]
+            npix = wl.n_x * wl.n_z
+            return wl, Job2D(config=wl.sta_config(), upsampling=8, coherence=CoherenceFactor(wl.cf, wl.cf_param),
+                             grid=wl.grid(), rx_apod=wl.rx_apodization(), echoes=d,
+                             value=torch.empty(npix, device="cuda"), factor=torch.empty(npix, device="cuda"),
+                             storage="f16")
+        for k in range(len(catalog)):   # warm the resident handles
+            assert eng.wait(eng.submit(job(k)[1]))["ok"]
+        for rate in [float(r) for r in args.rates.split(",")]:
+            picks = rng.integers(len(catalog), size=args.jobs)
+            gaps = rng.exponential(1.0 / rate, size=args.jobs)
+            pending, pairs, lat, busy = [], 0, [], 0
+            t0 = time.perf_counter()
+            t_next = t0
+            for k, g in zip(picks, gaps):
+                t_next += g
+                while time.perf_counter() < t_next:
+                    time.sleep(min(1e-3, max(0.0, t_next - time.perf_counter())))
+                wl, j = job(int(k))
+                try:
+                    pending.append((wl, j, eng.submit(j)))
+                except _lib.BusyError:
+                    busy += 1
+            for wl, j, t in pending:
+                s = eng.wait(t)
+                lat.append(s["latency_ms"])
+                pairs += wl.pixel_pairs
+            wall = time.perf_counter() - t0
+            results.append({"rate_fps": rate, "jobs": len(pending), "refused": busy,
+                            "gpixel_pair_per_s": round(pairs / wall / 1e9, 2),
+                            "frames_per_s": round(len(pending) / wall, 2),
+                            "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                            "p99_ms": round(float(np.percentile(lat, 99)), 2)})
+            print(f"[bench] rate {rate}: {results[-1]}", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "serving mix: Gpixel*pair/s and p50/p99 latency per arrival rate",
+                      "unit": "Gpixel*pair/s", "n_gpus": 1, "inflight": args.inflight, "storage": "f16",
+                      "data": "synthetic", "rates": results}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.mode == "batch":
+        return batch_mode(args)
+    if args.mode == "serve":
+        return serve_mode(args)
     import torch
     from us_lab4a_amd import imaging, workloads
     from us_lab4a_amd.sta import CoherenceFactor, HipSTAProcessor

@@ -205,6 +205,68 @@ int mista_set_kernel(mista_handle* h, int variant, int tile_param0, int tile_par
  * measured with HIP events on the launch stream); 0 if not available. */
 int mista_last_timing(const mista_handle* h, float* prep_ms, float* das_ms);
 
+/* ---- serving engine (no reference counterpart) ----
+ * A request loop for imaging jobs of uneven shapes on one GPU.  Jobs are
+ * admitted in arrival order as in-flight slots and device memory free up;
+ * each runs on a resident processor handle chosen by (configuration, grid,
+ * sample count), so delay tables, window tables and FFT plans stay in HBM
+ * between jobs of the same shape.  Admitted jobs run on separate streams and
+ * overlap on the device (continuous batching).  A job's images do not depend
+ * on the other jobs in flight: they are bit-identical to the same job run
+ * alone.  submit/wait/query may be called from any thread.
+ */
+typedef struct mista_engine mista_engine;
+
+typedef struct mista_engine_config {
+	int32_t device;
+	uint32_t max_inflight;       /* jobs on the device at once, 1..64          */
+	uint32_t max_queued;         /* jobs waiting for admission; more -> BUSY    */
+	uint32_t pad_;
+	uint64_t hbm_budget;         /* device bytes the resident handles may hold
+	                                (0 = 80 % of the free memory at creation)  */
+} mista_engine_config;
+
+/* One 2-D frame.  grid/apodization are copied at submit; echoes and outputs
+ * are used in place and must stay valid until mista_engine_wait returns.
+ * echoes: [ntx][nrx][ns] float32 on the device (echoes_on_device = 1) or on
+ * the host; value/factor: [n1*n2] float32 on the device (outputs_on_device =
+ * 1) or the host; factor may be NULL. */
+typedef struct mista_job {
+	mista_sta_config config;     /* device field ignored (the engine's device)  */
+	const float* grid_xz;        /* host [n1][n2][2]                            */
+	uint32_t n1, n2;
+	const float* rx_apod;        /* host [num_elements]                         */
+	const float* tx_apod;        /* host [ntx] or NULL                          */
+	const float* echoes;
+	uint32_t num_samples;
+	int32_t echoes_on_device;
+	float* value;
+	float* factor;
+	int32_t outputs_on_device;
+	uint32_t pad_;
+} mista_job;
+
+typedef struct mista_job_stats {
+	int32_t status;              /* MISTA_OK or the job's error code            */
+	uint32_t pad_;
+	double queued_ms;            /* submit -> admitted                          */
+	double latency_ms;           /* submit -> outputs ready (seen by the loop)  */
+	double device_ms;            /* job's work on its stream (HIP events)       */
+	uint64_t footprint;          /* device bytes of the handle that ran it      */
+} mista_job_stats;
+
+int mista_engine_create(const mista_engine_config* config, mista_engine** out);
+/* Finishes every submitted job, then frees all handles. */
+int mista_engine_destroy(mista_engine* e);
+/* Queues a job; MISTA_E_BUSY when max_queued jobs are already waiting. */
+int mista_engine_submit(mista_engine* e, const mista_job* job, uint64_t* ticket);
+/* Blocks until the job is done, fills stats, releases the ticket; returns the
+ * job's status. */
+int mista_engine_wait(mista_engine* e, uint64_t ticket, mista_job_stats* stats);
+int mista_engine_query(mista_engine* e, uint64_t ticket, int* done);
+int mista_engine_stats(mista_engine* e, uint64_t* resident_bytes, uint32_t* inflight, uint32_t* queued);
+const char* mista_engine_last_error(const mista_engine* e);
+
 /* ---- synthetic echoes (test and benchmark input) ----
  * Point scatterers (x, z, amplitude) stamped as a Gaussian-modulated pulse
  * exp(-a t^2) cos(2 pi fc t) at the round-trip time from element e_tx to

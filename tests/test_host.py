@@ -97,6 +97,21 @@ def test_create_3d_rejects_invalid_configuration(L, kw, code):
     assert L.mista_last_error(None)
 
 
+def test_engine_rejects_invalid_configuration(L):
+    h = ctypes.c_void_p()
+    for kw in (dict(max_inflight=0, max_queued=4), dict(max_inflight=65, max_queued=4), dict(max_inflight=2, max_queued=0)):
+        c = _lib.EngineConfig()
+        c.device = 0
+        for k, v in kw.items():
+            setattr(c, k, v)
+        assert L.mista_engine_create(ctypes.byref(c), ctypes.byref(h)) == _lib.MISTA_E_INVALID_PARAM
+        assert h.value is None
+    assert L.mista_engine_create(None, ctypes.byref(h)) == _lib.MISTA_E_INVALID_PARAM
+    assert L.mista_engine_destroy(None) == _lib.MISTA_OK
+    assert L.mista_engine_submit(None, None, None) == _lib.MISTA_E_INVALID_PARAM
+    assert L.mista_engine_wait(None, 1, None) == _lib.MISTA_E_INVALID_PARAM
+
+
 def test_null_handle_calls_fail_cleanly(L):
     assert L.mista_destroy(None) == 0
     assert L.mista_set_grid(None, None, 1, 1) == _lib.MISTA_E_INVALID_PARAM

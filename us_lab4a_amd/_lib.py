@@ -59,6 +59,48 @@ class Sta3dConfig(ctypes.Structure):
     ]
 
 
+class EngineConfig(ctypes.Structure):
+    """mista_engine_config (include/mista.h)."""
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("max_inflight", ctypes.c_uint32),
+        ("max_queued", ctypes.c_uint32),
+        ("pad_", ctypes.c_uint32),
+        ("hbm_budget", ctypes.c_uint64),
+    ]
+
+
+class Job(ctypes.Structure):
+    """mista_job (include/mista.h)."""
+    _fields_ = [
+        ("config", StaConfig),
+        ("grid_xz", ctypes.c_void_p),
+        ("n1", ctypes.c_uint32),
+        ("n2", ctypes.c_uint32),
+        ("rx_apod", ctypes.c_void_p),
+        ("tx_apod", ctypes.c_void_p),
+        ("echoes", ctypes.c_void_p),
+        ("num_samples", ctypes.c_uint32),
+        ("echoes_on_device", ctypes.c_int32),
+        ("value", ctypes.c_void_p),
+        ("factor", ctypes.c_void_p),
+        ("outputs_on_device", ctypes.c_int32),
+        ("pad_", ctypes.c_uint32),
+    ]
+
+
+class JobStats(ctypes.Structure):
+    """mista_job_stats (include/mista.h)."""
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("pad_", ctypes.c_uint32),
+        ("queued_ms", ctypes.c_double),
+        ("latency_ms", ctypes.c_double),
+        ("device_ms", ctypes.c_double),
+        ("footprint", ctypes.c_uint64),
+    ]
+
+
 class SynthParams(ctypes.Structure):
     """mista_synth_params (include/mista.h)."""
     _fields_ = [
@@ -120,6 +162,13 @@ SIGNATURES = {
     "mista_set_grid": (_i32, [_vp, _vp, _u32, _u32]),
     "mista_set_grid_points": (_i32, [_vp, _vp, _u32, _u32]),
     "mista_set_grid_xyz": (_i32, [_vp, _vp, _u32, _u32, _u32]),
+    "mista_engine_create": (_i32, [ctypes.POINTER(EngineConfig), ctypes.POINTER(_vp)]),
+    "mista_engine_destroy": (_i32, [_vp]),
+    "mista_engine_submit": (_i32, [_vp, ctypes.POINTER(Job), ctypes.POINTER(_u64)]),
+    "mista_engine_wait": (_i32, [_vp, _u64, ctypes.POINTER(JobStats)]),
+    "mista_engine_query": (_i32, [_vp, _u64, ctypes.POINTER(ctypes.c_int)]),
+    "mista_engine_stats": (_i32, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "mista_engine_last_error": (ctypes.c_char_p, [_vp]),
     "mista_create_3d": (_i32, [ctypes.POINTER(Sta3dConfig), _vp, _vp, ctypes.POINTER(_vp)]),
     "mista_set_elements_3d": (_i32, [_vp, _vp, _vp]),
     "mista_process": (_i32, [_vp, _vp, _u32, _vp, _vp]),

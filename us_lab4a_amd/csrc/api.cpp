@@ -341,6 +341,19 @@ void set_elements_3d(mista_handle* h, const float* tx_xy, const float* rx_xy)
 
 }  // namespace
 
+namespace mista {
+// Library-internal accessors used by the serving engine (serve.cpp).
+hipStream_t handle_stream(mista_handle* h) { return h->stream; }
+size_t handle_device_bytes(const mista_handle* h)
+{
+	return h->d_rx_apod.capacity() + h->d_tx_apod.capacity() + h->d_w.capacity() + h->d_elem.capacity() +
+			h->d_grid.capacity() + h->d_delay.capacity() + h->d_raw.capacity() + h->d_tensor.capacity() +
+			h->d_scale.capacity() + h->d_hpoly.capacity() + h->d_tmp.capacity() + h->d_value.capacity() +
+			h->d_factor.capacity() + h->das_scratch.partial.capacity() + h->das_scratch.bounds.capacity() +
+			h->das_scratch.params.capacity() + prep_device_bytes(h->prep);
+}
+}  // namespace mista
+
 extern "C" {
 
 int mista_abi_version(void) { return MISTA_ABI_VERSION; }

@@ -132,6 +132,10 @@ struct PrepState {
 void prepare_signals(PrepState& st, const FrameGeom& g, const float* d_raw, const float* d_filter_poly,
 		uint32_t taps_per_phase, uint32_t filter_offset, void* d_tensor, float* d_scale, hipStream_t s);
 
+// Device bytes held by the signal-preparation state (plans' buffers and an
+// estimate of the FFT library's work area).
+size_t prep_device_bytes(const PrepState& st);
+
 void launch_weights(const FrameGeom& g, const float* d_rx_apod, const float* d_tx_apod, const float* d_scale,
 		float* d_w, hipStream_t s);
 
@@ -156,5 +160,9 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 
 void launch_tensor_convert_in(const FrameGeom& g, const float* d_f32, void* d_tensor, float* d_scale, hipStream_t s);
 void launch_tensor_convert_out(const FrameGeom& g, const void* d_tensor, const float* d_scale, float* d_f32, hipStream_t s);
+
+// Serving-engine access to a processor handle (defined in api.cpp).
+hipStream_t handle_stream(mista_handle* h);
+size_t handle_device_bytes(const mista_handle* h);
 
 }  // namespace mista

@@ -398,6 +398,14 @@ struct PrepPlan {
 
 PrepState::~PrepState() { delete plan; }
 
+size_t prep_device_bytes(const PrepState& st)
+{
+	if (!st.plan) return 0;
+	const PrepPlan& P = *st.plan;
+	// the C2C plan's work area is taken as one more complex chunk
+	return P.real.capacity() + P.spec.capacity() + 2 * P.z.capacity();
+}
+
 static void make_plans(hipfftHandle* r2c, hipfftHandle* c2c, uint32_t nf, uint32_t batch, hipStream_t s)
 {
 	int n[1] = {static_cast<int>(nf)};
