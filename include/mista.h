@@ -201,6 +201,12 @@ int mista_query(const mista_handle* h, uint32_t* signal_length, uint32_t* fft_si
  * one-pixel-per-lane gather, 2 = LDS-tiled).  Tile parameters 0 = default. */
 int mista_set_kernel(mista_handle* h, int variant, int tile_param0, int tile_param1);
 
+/* Select the signal-preparation implementation: 0 = automatic (fused LDS
+ * kernel when the FFT fits on chip, else the library FFT), 1 = library FFT
+ * (FIR kernel, hipFFT R2C, spectrum kernel, hipFFT C2C, store kernel),
+ * 2 = fused LDS kernel (error if the frame does not fit). */
+int mista_set_prep(mista_handle* h, int method);
+
 /* Time of the last mista_das_device/mista_process_device DAS launch (ms,
  * measured with HIP events on the launch stream); 0 if not available. */
 int mista_last_timing(const mista_handle* h, float* prep_ms, float* das_ms);

@@ -80,10 +80,14 @@ def test_delay_table_fp64_rounded_once():
     p.close()
 
 
-def test_prepare_matches_oracle(small):
+@pytest.mark.parametrize("method", [1, 2], ids=["library_fft", "fused_lds"])
+def test_prepare_matches_oracle(small, method):
+    """Both preparation paths (hipFFT and the fused on-chip kernel) against
+    the FP64 oracle: interpolation, DC removal and analytic signal."""
     wl, echoes = small
-    say("prepare", wl.name)
+    say("prepare", wl.name, "method", method)
     p = make_proc(wl)
+    p.set_prep(method)
     p.process_echoes(echoes)
     a_gpu = p.download_analytic()
     a32 = oracle.prepare(wl.oracle_cfg(), echoes, np.float32)
@@ -277,3 +281,24 @@ def test_error_paths_raise_reference_style_errors():
     with pytest.raises(_lib.InvalidValueError, match="dead zone"):
         p.process_echoes(np.zeros((wl.num_tx, wl.num_elements, wl.num_samples), np.float32))
     p.close()
+
+
+@pytest.mark.parametrize("shape", [(16, 301, 1), (8, 1000, 3), (4, 4096, 8)], ids=["U1_odd", "U3", "U8_4096"])
+def test_fused_prepare_shapes(shape):
+    """Fused preparation at other FFT sizes (odd record, radix-3/5 heavy
+    sizes, the 4096-sample U = 8 channel of the 128-element frame) against
+    the FP64 oracle."""
+    N, ns, up = shape
+    wl = workloads.tiny(num_elements=N, num_samples=ns, upsampling=up)
+    echoes = wl.echoes_cpu()
+    say("fused prepare", shape)
+    p = make_proc(wl)
+    p.set_prep(2)
+    p.process_echoes(echoes)
+    a_gpu = p.download_analytic()
+    p.close()
+    a64 = oracle.prepare(wl.oracle_cfg(), echoes, np.float64)
+    m = np.abs(a64).max()
+    err = np.abs(a_gpu - a64).max() / m
+    say("fused prepare error / peak", err)
+    assert err <= 2e-6

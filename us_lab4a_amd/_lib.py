@@ -162,6 +162,7 @@ SIGNATURES = {
     "mista_set_grid": (_i32, [_vp, _vp, _u32, _u32]),
     "mista_set_grid_points": (_i32, [_vp, _vp, _u32, _u32]),
     "mista_set_grid_xyz": (_i32, [_vp, _vp, _u32, _u32, _u32]),
+    "mista_set_prep": (_i32, [_vp, _i32]),
     "mista_engine_create": (_i32, [ctypes.POINTER(EngineConfig), ctypes.POINTER(_vp)]),
     "mista_engine_destroy": (_i32, [_vp]),
     "mista_engine_submit": (_i32, [_vp, ctypes.POINTER(Job), ctypes.POINTER(_u64)]),

@@ -656,6 +656,15 @@ int mista_set_kernel(mista_handle* h, int variant, int p0, int p1)
 	});
 }
 
+int mista_set_prep(mista_handle* h, int method)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (method < 0 || method > 2) MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid preparation method " << method);
+		h->prep.method = method;
+	});
+}
+
 int mista_last_timing(const mista_handle* h, float* prep_ms, float* das_ms)
 {
 	if (!h) return MISTA_E_INVALID_PARAM;

@@ -124,6 +124,7 @@ void launch_delay_table_3d(const float* d_grid_xyz, uint64_t npix, const double*
 struct PrepPlan;  // hipFFT plans + scratch for signal preparation
 struct PrepState {
 	PrepPlan* plan = nullptr;  // owned; destroyed in prep.hip where PrepPlan is complete
+	int method = 0;            // 0 automatic, 1 library FFT (hipFFT), 2 fused LDS kernel
 	PrepState() = default;
 	PrepState(const PrepState&) = delete;
 	PrepState& operator=(const PrepState&) = delete;

@@ -230,6 +230,271 @@ __global__ void __launch_bounds__(256) hilbert_kernel(const float2* __restrict__
 	z[static_cast<uint64_t>(ch) * nf + k] = v;
 }
 
+// ---------------------------------------------------------------- fused LDS path
+
+// One workgroup per channel, everything between the raw echo and the
+// analytic signal on chip: FIR x U + DC removal into LDS, then the
+// analytic-signal FFTs of size N = Nf (HilbertEnvelope.h:142-251) as three
+// complex FFTs of size M = N/2 held in LDS:
+//   forward:  Z = DFT_M(x[2m] + i x[2m+1])            (real FFT of size N)
+//   spectrum: X[k] = E + W^k O, X[M-k] = conj(E - W^k O), W = exp(-2 pi i/N),
+//             E = (Z[k] + conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / (2i);
+//             bins 0 and M halved, bins above M dropped;
+//   inverse:  z[2m]   = (2/N) IDFT_M(Be)[m],  Be[k] = X[k], Be[0] = X[0]/2 + X[M]/2
+//             z[2m+1] = (2/N) IDFT_M(Bo)[m],  Bo[k] = X[k] conj(W^k), Bo[0] = X[0]/2 - X[M]/2
+// The inverses run as forward transforms of the conjugates.  FFTs are
+// Stockham passes of radix 8/4/2/3/5: every thread reads its butterflies'
+// inputs into registers, a barrier, then writes the outputs in place.
+// Bo waits in a global scratch row while Be is transformed.
+
+constexpr int kFftThreads = 1024;
+constexpr uint32_t kFftMaxM = 18432;
+constexpr int kFftMaxPasses = 16;
+
+struct FftPlanArgs {
+	uint32_t M, npass;
+	uint32_t radix[kFftMaxPasses];
+	uint32_t ns[kFftMaxPasses];        // sub-transform size before the pass
+	uint32_t tw_off[kFftMaxPasses];    // offset of the pass's twiddles in tw
+};
+
+struct AnalyticArgs {
+	const float* raw;      // [nch][ns], channels of this launch start here
+	const float* hpoly;    // [up][J]
+	const float2* tw;      // Stockham twiddles exp(-2 pi i * i * j / (Ns * R))
+	const float2* wpost;   // exp(-2 pi i k / N), k in [0, M/2]
+	float2* scratch;       // [blocks][M]
+	float2* out;           // [blocks][L]
+	uint32_t ns, up, J, off, L, N, dz;
+	FftPlanArgs plan;
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+	return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }   // -i a
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// In-register forward DFT of size R (sign -1).
+template<int R> __device__ __forceinline__ void dft(float2* v);
+template<> __device__ __forceinline__ void dft<2>(float2* v)
+{
+	const float2 a = v[0], b = v[1];
+	v[0] = cadd(a, b);
+	v[1] = csub(a, b);
+}
+template<> __device__ __forceinline__ void dft<4>(float2* v)
+{
+	const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+	const float2 s13 = cadd(v[1], v[3]), d13 = mul_mi(csub(v[1], v[3]));
+	v[0] = cadd(s02, s13);
+	v[2] = csub(s02, s13);
+	v[1] = cadd(d02, d13);
+	v[3] = csub(d02, d13);
+}
+template<> __device__ __forceinline__ void dft<8>(float2* v)
+{
+	float2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+	dft<4>(e);
+	dft<4>(o);
+	constexpr float c = 0.70710678118654752440f;
+	const float2 o1 = make_float2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x));     // * (c, -c)
+	const float2 o2 = mul_mi(o[2]);                                                  // * (0, -1)
+	const float2 o3 = make_float2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y));    // * (-c, -c)
+	v[0] = cadd(e[0], o[0]);
+	v[4] = csub(e[0], o[0]);
+	v[1] = cadd(e[1], o1);
+	v[5] = csub(e[1], o1);
+	v[2] = cadd(e[2], o2);
+	v[6] = csub(e[2], o2);
+	v[3] = cadd(e[3], o3);
+	v[7] = csub(e[3], o3);
+}
+template<> __device__ __forceinline__ void dft<3>(float2* v)
+{
+	constexpr float h = 0.86602540378443864676f;   // sin(2 pi / 3)
+	const float2 s = cadd(v[1], v[2]);
+	const float2 d = csub(v[1], v[2]);
+	const float2 t = make_float2(v[0].x - 0.5f * s.x, v[0].y - 0.5f * s.y);
+	const float2 u = make_float2(h * d.y, -h * d.x);   // -i h d
+	v[0] = cadd(v[0], s);
+	v[1] = cadd(t, u);
+	v[2] = csub(t, u);
+}
+template<> __device__ __forceinline__ void dft<5>(float2* v)
+{
+	constexpr float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+	constexpr float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+	const float2 b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+	const float2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+	const float2 t1 = make_float2(v[0].x + c1 * b1.x + c2 * b2.x, v[0].y + c1 * b1.y + c2 * b2.y);
+	const float2 t2 = make_float2(v[0].x + c2 * b1.x + c1 * b2.x, v[0].y + c2 * b1.y + c1 * b2.y);
+	const float2 u1 = make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
+	const float2 u2 = make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
+	v[0] = cadd(v[0], cadd(b1, b2));
+	v[1] = cadd(t1, mul_mi(u1));
+	v[4] = csub(t1, mul_mi(u1));
+	v[2] = cadd(t2, mul_mi(u2));
+	v[3] = csub(t2, mul_mi(u2));
+}
+
+template<int R>
+__device__ __forceinline__ void fft_pass(float2* D, uint32_t M, uint32_t Ns, const float2* __restrict__ tw)
+{
+	constexpr int MAXB = static_cast<int>((kFftMaxM / R + kFftThreads - 1) / kFftThreads);
+	const uint32_t nb = M / R;
+	float2 v[MAXB][R];
+#pragma unroll
+	for (int t = 0; t < MAXB; ++t) {
+		const uint32_t b = threadIdx.x + static_cast<uint32_t>(t) * kFftThreads;
+		if (b < nb) {
+#pragma unroll
+			for (int i = 0; i < R; ++i) v[t][i] = D[b + i * nb];
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int t = 0; t < MAXB; ++t) {
+		const uint32_t b = threadIdx.x + static_cast<uint32_t>(t) * kFftThreads;
+		if (b < nb) {
+			const uint32_t j = b % Ns;
+			const float2* w = tw + j * (R - 1);
+#pragma unroll
+			for (int i = 1; i < R; ++i) v[t][i] = cmul(v[t][i], w[i - 1]);
+			dft<R>(v[t]);
+			const uint32_t o = (b / Ns) * Ns * R + j;
+#pragma unroll
+			for (int i = 0; i < R; ++i) D[o + i * Ns] = v[t][i];
+		}
+	}
+	__syncthreads();
+}
+
+__device__ void fft_forward(float2* D, const FftPlanArgs& P, const float2* __restrict__ tw)
+{
+	for (uint32_t p = 0; p < P.npass; ++p) {
+		const float2* t = tw + P.tw_off[p];
+		switch (P.radix[p]) {
+		case 8: fft_pass<8>(D, P.M, P.ns[p], t); break;
+		case 4: fft_pass<4>(D, P.M, P.ns[p], t); break;
+		case 2: fft_pass<2>(D, P.M, P.ns[p], t); break;
+		case 3: fft_pass<3>(D, P.M, P.ns[p], t); break;
+		default: fft_pass<5>(D, P.M, P.ns[p], t); break;
+		}
+	}
+}
+
+__global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const AnalyticArgs A)
+{
+	extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+	__shared__ double red[kFftThreads / 64];
+	const uint32_t tid = threadIdx.x, ch = blockIdx.x;
+	const uint32_t M = A.plan.M, N = A.N, L = A.L, up = A.up, J = A.J;
+	float2* D = reinterpret_cast<float2*>(lds);
+	float* Df = reinterpret_cast<float*>(lds);
+	const uint32_t right = A.off / up + 2;
+	float* xs = reinterpret_cast<float*>(lds + static_cast<size_t>(M) * sizeof(float2));   // [J + ns + right]
+	float* hs = xs + J + A.ns + right;                                                    // [up][J]
+
+	// FIR x U (Interpolator.h:64-138, polyphase) and DC removal
+	// (Util.h:1009-1018) into the packed real array Df[0, L); Df[L, N) = 0.
+	const float* src = A.raw + static_cast<uint64_t>(ch) * A.ns;
+	if (up > 1) {
+		for (uint32_t i = tid; i < J + A.ns + right; i += kFftThreads) {
+			const int q = static_cast<int>(i) - static_cast<int>(J);
+			xs[i] = (q >= 0 && q < static_cast<int>(A.ns)) ? src[q] : 0.0f;
+		}
+		for (uint32_t i = tid; i < up * J; i += kFftThreads) hs[i] = A.hpoly[i];
+		__syncthreads();
+	}
+	double part = 0.0;
+	for (uint32_t i = tid; i < L; i += kFftThreads) {
+		float acc;
+		if (up > 1) {
+			const uint32_t m = i + A.off;
+			const float* hrow = hs + (m % up) * J;
+			const float* xrow = xs + J + m / up;
+			acc = 0.0f;
+			for (uint32_t j = 0; j < J; ++j) acc = fmaf(hrow[j], xrow[-static_cast<int>(j)], acc);
+		} else {
+			acc = src[i];
+		}
+		if (i >= A.dz) part += static_cast<double>(acc);
+		Df[i] = acc;
+	}
+	for (uint32_t i = L + tid; i < N; i += kFftThreads) Df[i] = 0.0f;
+	for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
+	if ((tid & 63) == 0) red[tid >> 6] = part;
+	__syncthreads();
+	if (tid == 0) {
+		double s = 0.0;
+		for (int w = 0; w < kFftThreads / 64; ++w) s += red[w];
+		red[0] = s;
+	}
+	__syncthreads();
+	const float mean = static_cast<float>(red[0] / static_cast<double>(L - A.dz));
+	for (uint32_t i = tid; i < L; i += kFftThreads) Df[i] = Df[i] - mean;
+	__syncthreads();
+
+	fft_forward(D, A.plan, A.tw);
+
+	// spectrum -> conj(Be) in LDS, conj(Bo) straight to the scratch row
+	float2* bo = A.scratch + static_cast<uint64_t>(ch) * M;
+	const uint32_t half = M / 2;
+	constexpr int kMaxPairs = static_cast<int>((kFftMaxM / 2 + kFftThreads) / kFftThreads);   // ceil((M/2+1)/threads)
+	float2 keep[kMaxPairs][2];
+#pragma unroll
+	for (int q = 0; q < kMaxPairs; ++q) {
+		const uint32_t k = tid + static_cast<uint32_t>(q) * kFftThreads;
+		if (k > half) continue;
+		const float2 a = D[k];
+		if (k == 0) {
+			keep[q][0] = make_float2(a.x, 0.0f);   // X[0]/2 + X[M]/2 = Re Z[0]
+			bo[0] = make_float2(a.y, 0.0f);        // X[0]/2 - X[M]/2 = Im Z[0]
+		} else {
+			const float2 b = D[M - k];
+			const float2 E = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+			const float2 O = mul_mi(make_float2(0.5f * (a.x - b.x), 0.5f * (a.y + b.y)));
+			const float2 w = A.wpost[k];
+			const float2 t = cmul(w, O);
+			const float2 xk = cadd(E, t);
+			const float2 xm = conjf2(csub(E, t));
+			keep[q][0] = conjf2(xk);
+			keep[q][1] = conjf2(xm);
+			bo[k] = conjf2(cmul(xk, conjf2(w)));
+			if (k != M - k) bo[M - k] = conjf2(cmul(xm, make_float2(-w.x, -w.y)));
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int q = 0; q < kMaxPairs; ++q) {
+		const uint32_t k = tid + static_cast<uint32_t>(q) * kFftThreads;
+		if (k > half) continue;
+		D[k] = keep[q][0];
+		if (k != 0 && k != M - k) D[M - k] = keep[q][1];
+	}
+	__syncthreads();
+
+	const float coef = 2.0f / static_cast<float>(N);
+	float2* dst = A.out + static_cast<uint64_t>(ch) * L;
+	fft_forward(D, A.plan, A.tw);
+	for (uint32_t m = tid; 2 * m < L; m += kFftThreads) {
+		const float2 v = D[m];
+		dst[2 * m] = make_float2(v.x * coef, -v.y * coef);
+	}
+	__syncthreads();
+	for (uint32_t m = tid; m < M; m += kFftThreads) D[m] = bo[m];
+	__syncthreads();
+	fft_forward(D, A.plan, A.tw);
+	for (uint32_t m = tid; 2 * m + 1 < L; m += kFftThreads) {
+		const float2 v = D[m];
+		dst[2 * m + 1] = make_float2(v.x * coef, -v.y * coef);
+	}
+}
+
 // ---------------------------------------------------------------- storage
 
 __device__ inline uint32_t pack_bf16x2(float re, float im)
@@ -387,6 +652,10 @@ struct PrepPlan {
 	hipfftHandle r2c = 0, c2c = 0, r2c_tail = 0, c2c_tail = 0;
 	bool has_tail = false;
 	DeviceBuffer real, spec, z;
+	// fused LDS path
+	uint32_t fused_nf = 0;
+	FftPlanArgs fplan{};
+	DeviceBuffer tw, wpost;
 	~PrepPlan()
 	{
 		if (r2c) hipfftDestroy(r2c);
@@ -403,7 +672,7 @@ size_t prep_device_bytes(const PrepState& st)
 	if (!st.plan) return 0;
 	const PrepPlan& P = *st.plan;
 	// the C2C plan's work area is taken as one more complex chunk
-	return P.real.capacity() + P.spec.capacity() + 2 * P.z.capacity();
+	return P.real.capacity() + P.spec.capacity() + 2 * P.z.capacity() + P.tw.capacity() + P.wpost.capacity();
 }
 
 static void make_plans(hipfftHandle* r2c, hipfftHandle* c2c, uint32_t nf, uint32_t batch, hipStream_t s)
@@ -429,12 +698,122 @@ static void launch_fir(uint32_t nch, const FrameGeom& g, const float* raw, const
 			hpoly, J, off, out);
 }
 
+// LDS bytes of the fused kernel for this frame (0 if it cannot run).
+static size_t fused_lds_bytes(const FrameGeom& g, uint32_t J, uint32_t off)
+{
+	if (g.nf % 2 != 0) return 0;
+	const uint32_t M = g.nf / 2;
+	if (M > kFftMaxM) return 0;
+	size_t b = static_cast<size_t>(M) * sizeof(float2);
+	if (g.up > 1) b += (static_cast<size_t>(J) + g.ns + off / g.up + 2 + static_cast<size_t>(g.up) * J) * sizeof(float);
+	return b <= 160 * 1024 - 512 ? b : 0;
+}
+
+// Radix plan (8, 4, 2, 3, 5 in that order) and twiddles of the size-M
+// transforms, computed in double and rounded once.
+static void build_fused_plan(PrepPlan& P, uint32_t nf, hipStream_t s)
+{
+	const uint32_t M = nf / 2;
+	FftPlanArgs F{};
+	F.M = M;
+	uint32_t m = M;
+	std::vector<uint32_t> rad;
+	while (m % 8 == 0) { rad.push_back(8); m /= 8; }
+	while (m % 4 == 0) { rad.push_back(4); m /= 4; }
+	while (m % 2 == 0) { rad.push_back(2); m /= 2; }
+	while (m % 3 == 0) { rad.push_back(3); m /= 3; }
+	while (m % 5 == 0) { rad.push_back(5); m /= 5; }
+	if (m != 1 || rad.size() > static_cast<size_t>(kFftMaxPasses))
+		MISTA_THROW(MISTA_E_INVALID_PARAM, "FFT size " << nf << " is not 2-3-5 smooth");
+	std::vector<float2> tw;
+	uint32_t Ns = 1;
+	for (size_t p = 0; p < rad.size(); ++p) {
+		const uint32_t R = rad[p];
+		F.radix[p] = R;
+		F.ns[p] = Ns;
+		F.tw_off[p] = static_cast<uint32_t>(tw.size());
+		for (uint32_t j = 0; j < Ns; ++j)
+			for (uint32_t i = 1; i < R; ++i) {
+				const double a = -2.0 * M_PI * static_cast<double>(i) * j / (static_cast<double>(Ns) * R);
+				tw.push_back(make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a))));
+			}
+		Ns *= R;
+	}
+	F.npass = static_cast<uint32_t>(rad.size());
+	tw.push_back(make_float2(0.0f, 0.0f));   // keep the table non-empty
+	std::vector<float2> wp(M / 2 + 1);
+	for (uint32_t k = 0; k <= M / 2; ++k) {
+		const double a = -2.0 * M_PI * static_cast<double>(k) / static_cast<double>(nf);
+		wp[k] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+	}
+	void* dt = P.tw.reserve(tw.size() * sizeof(float2));
+	void* dw = P.wpost.reserve(wp.size() * sizeof(float2));
+	HIP_CHECK(hipMemcpyAsync(dt, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice, s));
+	HIP_CHECK(hipMemcpyAsync(dw, wp.data(), wp.size() * sizeof(float2), hipMemcpyHostToDevice, s));
+	HIP_CHECK(hipStreamSynchronize(s));
+	P.fplan = F;
+	P.fused_nf = nf;
+}
+
+static void prepare_fused(PrepState& st, const FrameGeom& g, const float* d_raw, const float* d_filter_poly,
+		uint32_t J, uint32_t off, void* d_tensor, float* d_scale, size_t lds, hipStream_t s)
+{
+	const uint32_t nch = g.ntx * g.nrx;
+	const uint32_t chunk = std::min<uint32_t>(nch, 1024u);
+	if (!st.plan) st.plan = new PrepPlan();
+	PrepPlan& P = *st.plan;
+	if (P.fused_nf != g.nf) build_fused_plan(P, g.nf, s);
+	const uint32_t M = g.nf / 2;
+	float2* scratch = static_cast<float2*>(P.spec.reserve(static_cast<size_t>(chunk) * M * sizeof(float2)));
+	float2* tmp = (g.storage == MISTA_STORAGE_F32) ? nullptr
+			: static_cast<float2*>(P.z.reserve(static_cast<size_t>(chunk) * g.L * sizeof(float2)));
+	HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(analytic_lds_kernel),
+			hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+	AnalyticArgs A{};
+	A.hpoly = d_filter_poly;
+	A.tw = P.tw.as<const float2>();
+	A.wpost = P.wpost.as<const float2>();
+	A.scratch = scratch;
+	A.ns = g.ns;
+	A.up = g.up;
+	A.J = J;
+	A.off = off;
+	A.L = g.L;
+	A.N = g.nf;
+	A.dz = g.dz_up;
+	A.plan = P.fplan;
+	for (uint32_t c0 = 0; c0 < nch; c0 += chunk) {
+		const uint32_t n = std::min(chunk, nch - c0);
+		A.raw = d_raw + static_cast<uint64_t>(c0) * g.ns;
+		A.out = tmp ? tmp : static_cast<float2*>(d_tensor) + static_cast<uint64_t>(c0) * g.L;
+		hipLaunchKernelGGL(analytic_lds_kernel, dim3(n), dim3(kFftThreads), lds, s, A);
+		HIP_CHECK(hipGetLastError());
+		if (tmp) {
+			if (g.storage == MISTA_STORAGE_F16)
+				hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_F16>, dim3(n), dim3(256), 0, s, tmp, g.L, g.L, d_tensor, d_scale,
+						static_cast<uint64_t>(c0));
+			else
+				hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_BF16>, dim3(n), dim3(256), 0, s, tmp, g.L, g.L, d_tensor, d_scale,
+						static_cast<uint64_t>(c0));
+			HIP_CHECK(hipGetLastError());
+		}
+	}
+}
+
 void prepare_signals(PrepState& st, const FrameGeom& g, const float* d_raw, const float* d_filter_poly,
 		uint32_t J, uint32_t off, void* d_tensor, float* d_scale, hipStream_t s)
 {
 	const uint32_t nch = g.ntx * g.nrx;
+	if (st.method != 1) {
+		const size_t lds = fused_lds_bytes(g, J, off);
+		if (lds) {
+			prepare_fused(st, g, d_raw, d_filter_poly, J, off, d_tensor, d_scale, lds, s);
+			return;
+		}
+		if (st.method == 2) MISTA_THROW(MISTA_E_INVALID_PARAM, "the fused preparation does not fit this frame (FFT size " << g.nf << ")");
+	}
 	const uint32_t chunk = std::min<uint32_t>(nch, 1024u);
-	if (!st.plan || st.plan->nf != g.nf || st.plan->chunk != chunk || st.plan->tail != nch % chunk) {
+	if (!st.plan || !st.plan->r2c || st.plan->nf != g.nf || st.plan->chunk != chunk || st.plan->tail != nch % chunk) {
 		delete st.plan;
 		st.plan = nullptr;
 		auto p = std::make_unique<PrepPlan>();

@@ -151,6 +151,10 @@ class HipSTAProcessor:
         self._grid_shape = (n1, n2)
         self._grid_xz = xz
 
+    def set_prep(self, method):
+        """0 automatic, 1 library FFT (hipFFT), 2 fused LDS kernel."""
+        check(lib().mista_set_prep(self._h, method), self._h)
+
     def set_kernel(self, variant, p0=0, p1=0):
         check(lib().mista_set_kernel(self._h, variant, p0, p1), self._h)
 
