@@ -58,6 +58,19 @@ def algorithmic_bytes(wl, storage):
             + wl.num_elements * npix * 4 + npix * 4 * (1 + cf) + 4 * wl.num_elements)
 
 
+def measured_traffic(wl, storage):
+    """HBM-side bytes per DAS launch from the committed rocprofv3 counter
+    summary for this workload and storage (FETCH_SIZE + WRITE_SIZE, each
+    scaled by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"das_traffic_{wl.name}_{storage}.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(wl, threads):
     """Oracle (FP32 restatement of the reference TBB processor) on a bounded
     sample of the same frame: prep of a tx subset and DAS of a column subset,
@@ -285,6 +298,7 @@ def main():
     prep_avg = float(np.mean(prep_ms))
     alg = algorithmic_bytes(wl, args.storage)
     achieved = alg / (das_avg * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(wl, args.storage)
     value = wl.pixel_pairs / (ms_per_step * 1e-3) / 1e9
     out = {
         "metric": "TFM Gpixel*pair/s (end-to-end frame: prep + DAS + PCF + envelope)",
@@ -307,7 +321,8 @@ def main():
         "stage_ms": {"prep": round(prep_avg, 4), "das": round(das_avg, 4)},
         "das_gpixel_pair_per_s": round(wl.pixel_pairs / (das_avg * 1e-3) / 1e9, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "das (delay-and-sum + PCF + envelope)", "algorithmic_bytes": alg},
     }
     if not args.no_cpu_baseline:

@@ -281,9 +281,6 @@ das_tile_kernel(const TileArgs A)
 		if (c == 0)
 			lds_dma4(a.delay + static_cast<uint64_t>(r) * npix + p, __builtin_amdgcn_readfirstlane(bbase + win_bytes + wave * 256));
 	};
-	// copies issued by this wave for chunk k
-	auto copies = [&](uint32_t k) { return QW + ((k % CPR) == 0 ? 1u : 0u); };
-
 	// Windows of pairs with nothing to stage are read (with weight 0): start
 	// from zeros so that no stale bit pattern can be Inf/NaN.
 	for (uint32_t i = tid; i < NB * buf_bytes / 16; i += NT)
@@ -299,10 +296,10 @@ das_tile_kernel(const TileArgs A)
 		float drx = 0.0f, ar = 0.0f, ai = 0.0f;
 #pragma clang loop unroll(full)
 		for (int c = 0; c < static_cast<int>(CPR); ++c, ++i) {
-			// copies issued after chunk i: chunks i+1 .. min(i+NB-2, last)
-			uint32_t after = 0;
+			// copies issued after chunk i: chunks i+1 .. min(i+NB-2, last),
+			// QW each plus one for every chunk that starts an rx
 			const uint32_t hi = min(i + NB - 2, nchunks - 1);
-			for (uint32_t k = i + 1; k <= hi; ++k) after += copies(k);
+			const uint32_t after = (hi > i) ? (hi - i) * QW + (hi / CPR - i / CPR) : 0u;
 			wait_vmcnt_at_most(after);         // this wave's copies of chunk i landed
 			__syncthreads();                   // everyone's did; buffer of chunk i-1 is free
 			if (i + NB - 1 < nchunks) issue(i + NB - 1, (b + NB - 1) % NB);
@@ -325,7 +322,11 @@ das_tile_kernel(const TileArgs A)
 				const float pos = dtx[c * TC + j] + drx;
 				const float k = __builtin_amdgcn_fractf(pos);
 				const bool ok = pos >= 0.0f && pos < lim;
-				const float w = ok ? wr[j] : 0.0f;
+				// load the weight unconditionally (a select, not a branch
+				// around a scalar load)
+				float wv = wr[j];
+				asm volatile("" : "+s"(wv));
+				const float w = ok ? wv : 0.0f;
 				const float pc = __builtin_amdgcn_fmed3f(pos, 0.0f, lim);   // NaN-free, no int overflow
 				const int rel = min(max(static_cast<int>(pc) - ws, 0), static_cast<int>(wcap) - 2);
 				const int off = rel * static_cast<int>(ESZ) + static_cast<int>(bbase + j * wcap * ESZ);
