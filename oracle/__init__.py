@@ -128,6 +128,22 @@ def lib():
                                               _f32p, i32]
         L.oracle_das3d_f64.restype = i32
         L.oracle_das3d_f64.argtypes = [P3, _f64p, u32, _f32p, _f32p, _f32p, u32, u32, _f32p, _f64p, _f64p, i32]
+        L.oracle_interp4x_filter_f64.restype = i32
+        L.oracle_interp4x_filter_f64.argtypes = [_f64p]
+        L.oracle_interpolate4x_f64.restype = i32
+        L.oracle_interpolate4x_f64.argtypes = [_f64p, u32, _f64p]
+        L.oracle_prepare_rf_f32.restype = i32
+        L.oracle_prepare_rf_f32.argtypes = [P2, _f32p, _f32p, i32]
+        L.oracle_prepare_rf_f64.restype = i32
+        L.oracle_prepare_rf_f64.argtypes = [P2, _f32p, _f64p, i32]
+        L.oracle_das_rf_f32.restype = i32
+        L.oracle_das_rf_f32.argtypes = [P2, _f32p, u32, _f32p, u32, u32, _f32p, _f32p, i32]
+        L.oracle_das_rf_f64.restype = i32
+        L.oracle_das_rf_f64.argtypes = [P2, _f64p, u32, _f32p, u32, u32, _f64p, _f64p, i32]
+        L.oracle_envelope_dim2_f32.restype = i32
+        L.oracle_envelope_dim2_f32.argtypes = [_f32p, u32, u32, i32]
+        L.oracle_envelope_dim2_f64.restype = i32
+        L.oracle_envelope_dim2_f64.argtypes = [_f64p, u32, u32, i32]
         _lib = L
     return _lib
 
@@ -413,3 +429,51 @@ def das3d_rxsums(cfg, analytic_ri, tx_xy, rx_xy, grid_xyz, rx_apod, nthreads=0):
                                          np.ascontiguousarray(rx_xy, np.float32), grid_xyz, n1, n2,
                                          np.ascontiguousarray(rx_apod, np.float32), value, factor, sums, nthreads))
     return value, factor, sums[..., 0] + 1j * sums[..., 1]
+
+
+# ---------------------------------------------------------------- RF path
+
+def interp4x_filter():
+    h = np.empty(81, np.float64)
+    _check(lib().oracle_interp4x_filter_f64(h))
+    return h
+
+
+def interpolate4x(x):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty(4 * x.size, np.float64)
+    _check(lib().oracle_interpolate4x_f64(x, x.size, out))
+    return out
+
+
+def prepare_rf(cfg, echoes, dtype=np.float32, nthreads=0):
+    """DefaultSTAProcessor preparation: real [ntx][nrx][4 ns]."""
+    echoes = np.ascontiguousarray(echoes, np.float32)
+    ntx, nrx, ns = echoes.shape
+    out = np.empty((ntx, nrx, 4 * ns), dtype)
+    fn = lib().oracle_prepare_rf_f32 if dtype == np.float32 else lib().oracle_prepare_rf_f64
+    _check(fn(ctypes.byref(cfg), echoes, out, nthreads))
+    return out
+
+
+def das_rf(cfg, tensor, grid_xz, nthreads=0):
+    tensor = np.ascontiguousarray(tensor)
+    L = tensor.shape[2]
+    grid_xz = np.ascontiguousarray(grid_xz, np.float32)
+    n1, n2 = grid_xz.shape[:2]
+    value = np.empty((n1, n2), tensor.dtype)
+    factor = np.empty((n1, n2), tensor.dtype)
+    fn = lib().oracle_das_rf_f32 if tensor.dtype == np.float32 else lib().oracle_das_rf_f64
+    _check(fn(ctypes.byref(cfg), tensor, L, grid_xz, n1, n2, value, factor, nthreads))
+    return value, factor
+
+
+def envelope_dim2(data, nthreads=0):
+    """Magnitude of the analytic signal of every row (n1 x n2)."""
+    d = np.array(data, copy=True, order="C")
+    fn = lib().oracle_envelope_dim2_f32 if d.dtype == np.float32 else lib().oracle_envelope_dim2_f64
+    if d.dtype not in (np.float32, np.float64):
+        d = d.astype(np.float64)
+        fn = lib().oracle_envelope_dim2_f64
+    _check(fn(d, d.shape[0], d.shape[1], nthreads))
+    return d

@@ -419,6 +419,129 @@ void prepare(uint32_t ntx, uint32_t nrx, uint32_t ns, uint32_t up, unsigned dz_u
 	}
 }
 
+/* ---------------- RF path (DefaultSTAProcessor) ---------------- */
+
+/* Interpolator4X.h:54-72 filter, from the recipe stated there: 81 taps,
+ * h = sinc(w) .* kaiser(81, beta = 5), w = linspace(-10, 10, 81). */
+template<typename T>
+void interp4x_filter(std::vector<T>& h)
+{
+	const unsigned n = 81;
+	std::vector<double> win;
+	kaiser_window<double>(n, 5.0, win);
+	h.resize(n);
+	for (unsigned i = 0; i < n; ++i) {
+		const double w = -10.0 + 20.0 * i / (n - 1.0);
+		const double s = (w == 0.0) ? 1.0 : std::sin(M_PI * w) / (M_PI * w);
+		h[i] = static_cast<T>(s * win[i]);
+	}
+}
+
+/* DefaultSTAProcessor.h:136-164 for one frame: Interpolator4X (zero-stuff x4,
+ * 81-tap filter, output aligned by 40) then removeDC over [dz, L). Output
+ * real [ntx][nrx][4 ns]. */
+template<typename T>
+void prepare_rf(uint32_t ntx, uint32_t nrx, uint32_t ns, unsigned dz_up, const float* echoes, T* out, int nthreads)
+{
+	const unsigned L = ns * 4;
+	if (dz_up >= L) throw std::domain_error("Invalid dead zone: deadZoneSamplesUp >= signalLength");
+	std::vector<T> h;
+	interp4x_filter<T>(h);
+	const long nch = static_cast<long>(ntx) * nrx;
+	const int nt = num_threads_or_default<T>(nthreads);
+#pragma omp parallel num_threads(nt)
+	{
+		std::vector<T> in(ns);
+#pragma omp for schedule(dynamic, 1)
+		for (long ch = 0; ch < nch; ++ch) {
+			const float* src = echoes + static_cast<size_t>(ch) * ns;
+			for (unsigned i = 0; i < ns; ++i) in[i] = static_cast<T>(src[i]);
+			T* dst = out + static_cast<size_t>(ch) * L;
+			interpolate<T>(in.data(), ns, 4, h, dst);
+			remove_dc<T>(dst, L, dz_up);
+		}
+	}
+}
+
+/* DefaultSTAProcessor.h:166-226: real per-rx sums (no apodization), value =
+ * signed sum / (Ntx * Nrx), factor = SCF of the per-rx sums (or 1).  A
+ * position below zero is skipped (the reference's size_t cast then fails
+ * the i + 1 < L test for any realistic record). */
+template<typename T>
+void das_rf(const oracle_sta2d& cfg, const T* tensor, uint32_t L, const float* grid_xz, uint32_t n1, uint32_t n2,
+		T* value, T* factor, int nthreads)
+{
+	const unsigned N = cfg.num_elements;
+	const unsigned first = cfg.first_tx_elem, last = cfg.last_tx_elem;
+	const unsigned ntx = last - first + 1U;
+	std::vector<T> xa(N);
+	const T pitch = static_cast<T>(cfg.pitch);
+	const T xc = ((N - 1) * pitch) / 2;
+	for (unsigned e = 0; e < N; ++e) xa[e] = e * pitch - xc;
+	const T fs = static_cast<T>(cfg.sampling_frequency);
+	const T c = static_cast<T>(cfg.propagation_speed);
+	const T fc = static_cast<T>(cfg.center_frequency);
+	const T invCT = (fs * 4) / c;
+	const T offset = (fs * 4) * static_cast<T>(cfg.peak_offset) / fc;
+	const size_t num_signals = static_cast<size_t>(ntx) * N;
+	const int cf = cfg.cf_kind;
+	const T cf_param = static_cast<T>(cfg.cf_param);
+	const int nt = num_threads_or_default<T>(nthreads);
+#pragma omp parallel num_threads(nt)
+	{
+		std::vector<T> sums(N), delay(N);
+#pragma omp for schedule(dynamic, 1)
+		for (long col = 0; col < static_cast<long>(n1); ++col) {
+			for (unsigned row = 0; row < n2; ++row) {
+				const size_t pt = static_cast<size_t>(col) * n2 + row;
+				const T x = grid_xz[2 * pt], z = grid_xz[2 * pt + 1];
+				for (unsigned e = 0; e < N; ++e) {
+					const T dx = x - xa[e];
+					delay[e] = std::sqrt(dx * dx + z * z) * invCT;
+				}
+				std::fill(sums.begin(), sums.end(), T(0));
+				for (unsigned t = first; t <= last; ++t) {
+					const T txd = delay[t];
+					for (unsigned r = 0; r < N; ++r) {
+						const T d = offset + txd + delay[r];
+						if (d < 0) continue;
+						const size_t idx = static_cast<size_t>(d);
+						const T k = d - idx;
+						if (idx + 1U < L) {
+							const T* p = tensor + (static_cast<size_t>(t - first) * N + r) * L + idx;
+							sums[r] += (1 - k) * p[0] + k * p[1];
+						}
+					}
+				}
+				factor[pt] = (cf == ORACLE_CF_SCF) ? scf<T>(sums.data(), N, cf_param) : T(1);
+				value[pt] = std::accumulate(sums.begin(), sums.end(), T(0));
+			}
+		}
+	}
+	const T scale = T(1) / num_signals;
+	for (size_t i = 0; i < static_cast<size_t>(n1) * n2; ++i) value[i] *= scale;
+}
+
+/* ParallelHilbertEnvelope::calculateDim2 (ParallelHilbertEnvelope.h:73-83)
+ * with HilbertEnvelope::calculate (HilbertEnvelope.h:201-223): every row of
+ * the n1 x n2 matrix replaced by the magnitude of its analytic signal. */
+template<typename T>
+void envelope_dim2(T* data, uint32_t n1, uint32_t n2, int nthreads)
+{
+	const int nt = num_threads_or_default<T>(nthreads);
+#pragma omp parallel num_threads(nt)
+	{
+		Analytic<T> an(n2);
+		std::vector<T> z(2 * static_cast<size_t>(n2));
+#pragma omp for schedule(dynamic, 1)
+		for (long i = 0; i < static_cast<long>(n1); ++i) {
+			T* row = data + static_cast<size_t>(i) * n2;
+			an.run(row, z.data());
+			for (unsigned j = 0; j < n2; ++j) row[j] = std::sqrt(z[2 * j] * z[2 * j] + z[2 * j + 1] * z[2 * j + 1]);
+		}
+	}
+}
+
 /* VectorialSTAProcessor.h:237-338.  `T` is the processor float type. */
 template<typename T>
 void das(const oracle_sta2d& cfg, const T* analytic_ri, uint32_t L,
@@ -867,6 +990,64 @@ int oracle_das3d_rxsums_f32(const oracle_sta3d* cfg, const float* analytic, uint
 	return guarded([&] {
 		das3d<float>(*cfg, analytic, L, tx_xy, rx_xy, grid_xyz, n1, n2, rx_apod, value, factor, nthreads, rxsums);
 	});
+}
+
+int oracle_interp4x_filter_f64(double* h)
+{
+	return guarded([&] {
+		std::vector<double> v;
+		interp4x_filter<double>(v);
+		std::copy(v.begin(), v.end(), h);
+	});
+}
+
+int oracle_interpolate4x_f64(const double* x, uint32_t n, double* out)
+{
+	return guarded([&] {
+		std::vector<double> h;
+		interp4x_filter<double>(h);
+		interpolate<double>(x, n, 4, h, out);
+	});
+}
+
+int oracle_prepare_rf_f32(const oracle_sta2d* cfg, const float* echoes, float* out, int nthreads)
+{
+	return guarded([&] {
+		const unsigned dz = dead_zone_samples(4, cfg->sampling_frequency, cfg->dead_zone_m, cfg->propagation_speed, true);
+		prepare_rf<float>(cfg->last_tx_elem - cfg->first_tx_elem + 1, cfg->num_elements, cfg->num_samples, dz, echoes,
+				out, nthreads);
+	});
+}
+
+int oracle_prepare_rf_f64(const oracle_sta2d* cfg, const float* echoes, double* out, int nthreads)
+{
+	return guarded([&] {
+		const unsigned dz = dead_zone_samples(4, cfg->sampling_frequency, cfg->dead_zone_m, cfg->propagation_speed, false);
+		prepare_rf<double>(cfg->last_tx_elem - cfg->first_tx_elem + 1, cfg->num_elements, cfg->num_samples, dz, echoes,
+				out, nthreads);
+	});
+}
+
+int oracle_das_rf_f32(const oracle_sta2d* cfg, const float* tensor, uint32_t L, const float* grid_xz, uint32_t n1,
+		uint32_t n2, float* value, float* factor, int nthreads)
+{
+	return guarded([&] { das_rf<float>(*cfg, tensor, L, grid_xz, n1, n2, value, factor, nthreads); });
+}
+
+int oracle_das_rf_f64(const oracle_sta2d* cfg, const double* tensor, uint32_t L, const float* grid_xz, uint32_t n1,
+		uint32_t n2, double* value, double* factor, int nthreads)
+{
+	return guarded([&] { das_rf<double>(*cfg, tensor, L, grid_xz, n1, n2, value, factor, nthreads); });
+}
+
+int oracle_envelope_dim2_f32(float* data, uint32_t n1, uint32_t n2, int nthreads)
+{
+	return guarded([&] { envelope_dim2<float>(data, n1, n2, nthreads); });
+}
+
+int oracle_envelope_dim2_f64(double* data, uint32_t n1, uint32_t n2, int nthreads)
+{
+	return guarded([&] { envelope_dim2<double>(data, n1, n2, nthreads); });
 }
 
 int oracle_das3d_f64(const oracle_sta3d* cfg, const double* analytic, uint32_t L,

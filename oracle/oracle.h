@@ -139,6 +139,20 @@ int oracle_das3d_f64(const oracle_sta3d* cfg, const double* analytic, uint32_t s
 		const float* grid_xyz, uint32_t n1, uint32_t n2,
 		const float* rx_apod, double* value, double* factor, int nthreads);
 
+/* RF path (DefaultSTAProcessor.h:136-226, Interpolator4X.h,
+ * ParallelHilbertEnvelope.h:73-83).  Upsampling is fixed at 4; the tensor is
+ * real [ntx][nrx][4 ns]. */
+int oracle_interp4x_filter_f64(double* h /* [81] */);
+int oracle_interpolate4x_f64(const double* x, uint32_t n, double* out /* [4 n] */);
+int oracle_prepare_rf_f32(const oracle_sta2d* cfg, const float* echoes, float* out, int nthreads);
+int oracle_prepare_rf_f64(const oracle_sta2d* cfg, const float* echoes, double* out, int nthreads);
+int oracle_das_rf_f32(const oracle_sta2d* cfg, const float* tensor, uint32_t signal_length, const float* grid_xz,
+		uint32_t n1, uint32_t n2, float* value, float* factor, int nthreads);
+int oracle_das_rf_f64(const oracle_sta2d* cfg, const double* tensor, uint32_t signal_length, const float* grid_xz,
+		uint32_t n1, uint32_t n2, double* value, double* factor, int nthreads);
+int oracle_envelope_dim2_f32(float* data, uint32_t n1, uint32_t n2, int nthreads);
+int oracle_envelope_dim2_f64(double* data, uint32_t n1, uint32_t n2, int nthreads);
+
 const char* oracle_last_error(void);
 
 #ifdef __cplusplus
