@@ -201,6 +201,27 @@ int mista_query(const mista_handle* h, uint32_t* signal_length, uint32_t* fft_si
  * one-pixel-per-lane gather, 2 = LDS-tiled).  Tile parameters 0 = default. */
 int mista_set_kernel(mista_handle* h, int variant, int tile_param0, int tile_param1);
 
+/*
+ * RF processor: replaces DefaultSTAProcessor (DefaultSTAProcessor.h:50-226;
+ * constructor (config, acquisition, coherenceFactor, peakOffset)).  Echoes are
+ * upsampled by 4 with Interpolator4X's fixed 81-tap filter (Interpolator4X.h),
+ * DC removed, and summed as real samples without apodization; value is the
+ * signed sum / (Ntx*Nrx) and factor the sign coherence factor of the per-rx
+ * sums (cf_kind MISTA_CF_NONE or MISTA_CF_SCF; upsampling_factor 4 or 0;
+ * storage ignored).  The envelope that STAMethod takes afterwards
+ * (ParallelHilbertEnvelope::calculateDim2, STAMethod.cpp:63-72) is
+ * mista_envelope / mista_envelope_device.  mista_upload/download_analytic
+ * move the real tensor [ntx][nrx][4 ns] on an RF handle.
+ */
+int mista_create_rf(const mista_sta_config* config, mista_handle** out);
+
+/* ParallelHilbertEnvelope::calculateDim2 (ParallelHilbertEnvelope.h:73-83)
+ * with HilbertEnvelope::calculate (HilbertEnvelope.h:201-223): every row of
+ * the n1 x n2 float matrix (n2 contiguous) replaced, in place, by the
+ * magnitude of its analytic signal.  Any handle can run it. */
+int mista_envelope_device(mista_handle* h, float* d_data, uint32_t n1, uint32_t n2, void* stream);
+int mista_envelope(mista_handle* h, float* data, uint32_t n1, uint32_t n2);
+
 /* Select the signal-preparation implementation: 0 = automatic (fused LDS
  * kernel when the FFT fits on chip, else the library FFT), 1 = library FFT
  * (FIR kernel, hipFFT R2C, spectrum kernel, hipFFT C2C, store kernel),

@@ -138,6 +138,8 @@ def lib():
         L.oracle_prepare_rf_f64.argtypes = [P2, _f32p, _f64p, i32]
         L.oracle_das_rf_f32.restype = i32
         L.oracle_das_rf_f32.argtypes = [P2, _f32p, u32, _f32p, u32, u32, _f32p, _f32p, i32]
+        L.oracle_das_rf_rxsums_f32.restype = i32
+        L.oracle_das_rf_rxsums_f32.argtypes = [P2, _f32p, u32, _f32p, u32, u32, _f32p, _f32p, _f32p, i32]
         L.oracle_das_rf_f64.restype = i32
         L.oracle_das_rf_f64.argtypes = [P2, _f64p, u32, _f32p, u32, u32, _f64p, _f64p, i32]
         L.oracle_envelope_dim2_f32.restype = i32
@@ -466,6 +468,19 @@ def das_rf(cfg, tensor, grid_xz, nthreads=0):
     fn = lib().oracle_das_rf_f32 if tensor.dtype == np.float32 else lib().oracle_das_rf_f64
     _check(fn(ctypes.byref(cfg), tensor, L, grid_xz, n1, n2, value, factor, nthreads))
     return value, factor
+
+
+def das_rf_rxsums(cfg, tensor, grid_xz, nthreads=0):
+    """FP32 RF DAS also returning the real per-rx sums [n1][n2][num_elements]."""
+    tensor = np.ascontiguousarray(tensor, np.float32)
+    L = tensor.shape[2]
+    grid_xz = np.ascontiguousarray(grid_xz, np.float32)
+    n1, n2 = grid_xz.shape[:2]
+    value = np.empty((n1, n2), np.float32)
+    factor = np.empty((n1, n2), np.float32)
+    sums = np.empty((n1, n2, cfg.num_elements), np.float32)
+    _check(lib().oracle_das_rf_rxsums_f32(ctypes.byref(cfg), tensor, L, grid_xz, n1, n2, value, factor, sums, nthreads))
+    return value, factor, sums
 
 
 def envelope_dim2(data, nthreads=0):

@@ -469,7 +469,7 @@ void prepare_rf(uint32_t ntx, uint32_t nrx, uint32_t ns, unsigned dz_up, const f
  * the i + 1 < L test for any realistic record). */
 template<typename T>
 void das_rf(const oracle_sta2d& cfg, const T* tensor, uint32_t L, const float* grid_xz, uint32_t n1, uint32_t n2,
-		T* value, T* factor, int nthreads)
+		T* value, T* factor, int nthreads, T* rxsum_out = nullptr)
 {
 	const unsigned N = cfg.num_elements;
 	const unsigned first = cfg.first_tx_elem, last = cfg.last_tx_elem;
@@ -513,6 +513,7 @@ void das_rf(const oracle_sta2d& cfg, const T* tensor, uint32_t L, const float* g
 						}
 					}
 				}
+				if (rxsum_out) std::copy(sums.begin(), sums.end(), rxsum_out + pt * N);
 				factor[pt] = (cf == ORACLE_CF_SCF) ? scf<T>(sums.data(), N, cf_param) : T(1);
 				value[pt] = std::accumulate(sums.begin(), sums.end(), T(0));
 			}
@@ -1032,6 +1033,12 @@ int oracle_das_rf_f32(const oracle_sta2d* cfg, const float* tensor, uint32_t L, 
 		uint32_t n2, float* value, float* factor, int nthreads)
 {
 	return guarded([&] { das_rf<float>(*cfg, tensor, L, grid_xz, n1, n2, value, factor, nthreads); });
+}
+
+int oracle_das_rf_rxsums_f32(const oracle_sta2d* cfg, const float* tensor, uint32_t L, const float* grid_xz, uint32_t n1,
+		uint32_t n2, float* value, float* factor, float* rxsums, int nthreads)
+{
+	return guarded([&] { das_rf<float>(*cfg, tensor, L, grid_xz, n1, n2, value, factor, nthreads, rxsums); });
 }
 
 int oracle_das_rf_f64(const oracle_sta2d* cfg, const double* tensor, uint32_t L, const float* grid_xz, uint32_t n1,

@@ -148,6 +148,8 @@ int oracle_prepare_rf_f32(const oracle_sta2d* cfg, const float* echoes, float* o
 int oracle_prepare_rf_f64(const oracle_sta2d* cfg, const float* echoes, double* out, int nthreads);
 int oracle_das_rf_f32(const oracle_sta2d* cfg, const float* tensor, uint32_t signal_length, const float* grid_xz,
 		uint32_t n1, uint32_t n2, float* value, float* factor, int nthreads);
+int oracle_das_rf_rxsums_f32(const oracle_sta2d* cfg, const float* tensor, uint32_t signal_length, const float* grid_xz,
+		uint32_t n1, uint32_t n2, float* value, float* factor, float* rxsums, int nthreads);
 int oracle_das_rf_f64(const oracle_sta2d* cfg, const double* tensor, uint32_t signal_length, const float* grid_xz,
 		uint32_t n1, uint32_t n2, double* value, double* factor, int nthreads);
 int oracle_envelope_dim2_f32(float* data, uint32_t n1, uint32_t n2, int nthreads);

@@ -163,6 +163,9 @@ SIGNATURES = {
     "mista_set_grid_points": (_i32, [_vp, _vp, _u32, _u32]),
     "mista_set_grid_xyz": (_i32, [_vp, _vp, _u32, _u32, _u32]),
     "mista_set_prep": (_i32, [_vp, _i32]),
+    "mista_create_rf": (_i32, [ctypes.POINTER(StaConfig), ctypes.POINTER(_vp)]),
+    "mista_envelope_device": (_i32, [_vp, _vp, _u32, _u32, _vp]),
+    "mista_envelope": (_i32, [_vp, _vp, _u32, _u32]),
     "mista_engine_create": (_i32, [ctypes.POINTER(EngineConfig), ctypes.POINTER(_vp)]),
     "mista_engine_destroy": (_i32, [_vp]),
     "mista_engine_submit": (_i32, [_vp, ctypes.POINTER(Job), ctypes.POINTER(_u64)]),

@@ -35,6 +35,8 @@ struct mista_handle {
 	// delay-table rows: 2-D one per array element (tx t at row first_tx + t);
 	// 3-D active rx elements, then active tx elements (positions in elem_xy)
 	bool is3d = false;
+	bool rf = false;               // DefaultSTAProcessor: real samples, fixed 4x filter
+	PrepState env;                 // row-envelope FFT plan (its own size)
 	uint32_t rows = 0, tx_row0 = 0;
 	std::vector<double> elem_xy;   // 3-D: [rows][2]
 	DeviceBuffer d_elem;
@@ -170,8 +172,9 @@ void ensure_filter(mista_handle* h, hipStream_t s)
 {
 	const uint32_t up = h->cfg.upsampling_factor;
 	if (up < 2 || h->filter_up == up) return;
-	// VectorialSTAProcessor.h:92 upsampFilterHalfTransitionWidth = 0.2
-	const std::vector<double> taps = upsampling_filter(up, 0.2);
+	// RF: Interpolator4X's fixed 81-tap filter; otherwise the Kaiser design
+	// with VectorialSTAProcessor.h:92 upsampFilterHalfTransitionWidth = 0.2
+	const std::vector<double> taps = h->rf ? rf4x_filter() : upsampling_filter(up, 0.2);
 	const uint32_t T = static_cast<uint32_t>(taps.size());
 	const uint32_t J = (T + up - 1) / up;
 	std::vector<float> poly(static_cast<size_t>(up) * J, 0.0f);
@@ -222,8 +225,12 @@ void run_prepare(mista_handle* h, const float* d_echoes, uint32_t ns, hipStream_
 	ensure_filter(h, s);
 	ensure_events(h);
 	HIP_CHECK(hipEventRecord(h->ev[0], s));
-	prepare_signals(h->prep, h->geom, d_echoes, h->d_hpoly.as<const float>(), h->filter_J, h->filter_off,
-			h->d_tensor.get(), h->d_scale.as<float>(), s);
+	if (h->rf)
+		prepare_signals_rf(h->geom, d_echoes, h->d_hpoly.as<const float>(), h->filter_J, h->filter_off,
+				h->d_tensor.as<float>(), s);
+	else
+		prepare_signals(h->prep, h->geom, d_echoes, h->d_hpoly.as<const float>(), h->filter_J, h->filter_off,
+				h->d_tensor.get(), h->d_scale.as<float>(), s);
 	HIP_CHECK(hipEventRecord(h->ev[1], s));
 	h->signal_ready = true;
 }
@@ -427,6 +434,63 @@ int mista_create_3d(const mista_sta3d_config* config, const float* tx_xy, const 
 	});
 }
 
+int mista_create_rf(const mista_sta_config* config, mista_handle** out)
+{
+	if (!config || !out) {
+		g_thread_error = "null argument";
+		return MISTA_E_INVALID_PARAM;
+	}
+	*out = nullptr;
+	return guarded(nullptr, [&] {
+		mista_sta_config c = *config;
+		if (c.upsampling_factor == 0) c.upsampling_factor = 4;
+		if (c.upsampling_factor != 4)
+			MISTA_THROW(MISTA_E_INVALID_VALUE, "The RF processor upsamples by 4 (got " << c.upsampling_factor << ").");
+		if (c.cf_kind == MISTA_CF_PCF)
+			MISTA_THROW(MISTA_E_INVALID_PARAM, "The RF processor supports no coherence factor or the sign coherence factor.");
+		c.storage = MISTA_STORAGE_F32;
+		validate(c);
+		int ndev = 0;
+		HIP_CHECK(hipGetDeviceCount(&ndev));
+		if (c.device < 0 || c.device >= ndev)
+			MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid device " << c.device << " (" << ndev << " devices).");
+		HIP_CHECK(hipSetDevice(c.device));
+		auto h = std::make_unique<mista_handle>();
+		c.storage = kStorageReal;
+		h->cfg = c;
+		h->rf = true;
+		h->rows = c.num_elements;
+		h->tx_row0 = c.first_tx_elem;
+		HIP_CHECK(hipStreamCreate(&h->stream));  // default priority
+		h->rx_apod.assign(c.num_elements, 1.0f);
+		*out = h.release();
+	});
+}
+
+int mista_envelope_device(mista_handle* h, float* d_data, uint32_t n1, uint32_t n2, void* stream)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!d_data) MISTA_THROW(MISTA_E_INVALID_PARAM, "null device pointer.");
+		envelope_rows(h->env, d_data, n1, n2, pick_stream(h, stream));
+	});
+}
+
+int mista_envelope(mista_handle* h, float* data, uint32_t n1, uint32_t n2)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!data) MISTA_THROW(MISTA_E_INVALID_PARAM, "null host pointer.");
+		hipStream_t s = h->stream;
+		const size_t bytes = static_cast<size_t>(n1) * n2 * sizeof(float);
+		float* d = static_cast<float*>(h->d_tmp.reserve(bytes));
+		HIP_CHECK(hipMemcpyAsync(d, data, bytes, hipMemcpyHostToDevice, s));
+		envelope_rows(h->env, d, n1, n2, s);
+		HIP_CHECK(hipMemcpyAsync(data, d, bytes, hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+	});
+}
+
 int mista_set_elements_3d(mista_handle* h, const float* tx_xy, const float* rx_xy)
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
@@ -597,6 +661,13 @@ int mista_upload_analytic(mista_handle* h, const float* analytic_ri, uint32_t si
 		hipStream_t s = h->stream;
 		set_frame_geom(h, 0, signal_length, s);
 		const size_t nch = static_cast<size_t>(h->geom.ntx) * h->geom.nrx;
+		if (h->rf) {   // real samples [ntx][nrx][L]
+			HIP_CHECK(hipMemcpyAsync(h->d_tensor.get(), analytic_ri, nch * signal_length * sizeof(float),
+					hipMemcpyHostToDevice, s));
+			HIP_CHECK(hipStreamSynchronize(s));
+			h->signal_ready = true;
+			return;
+		}
 		const size_t bytes = nch * signal_length * 2 * sizeof(float);
 		float* d_tmp = static_cast<float*>(h->d_tmp.reserve(bytes));
 		HIP_CHECK(hipMemcpyAsync(d_tmp, analytic_ri, bytes, hipMemcpyHostToDevice, s));
@@ -615,6 +686,11 @@ int mista_download_analytic(mista_handle* h, float* analytic_ri)
 		if (!h->signal_ready) MISTA_THROW(MISTA_E_INVALID_STATE, "No prepared signals.");
 		hipStream_t s = h->stream;
 		const size_t nch = static_cast<size_t>(h->geom.ntx) * h->geom.nrx;
+		if (h->rf) {
+			HIP_CHECK(hipMemcpyAsync(analytic_ri, h->d_tensor.get(), nch * h->geom.L * sizeof(float), hipMemcpyDeviceToHost, s));
+			HIP_CHECK(hipStreamSynchronize(s));
+			return;
+		}
 		const size_t bytes = nch * h->geom.L * 2 * sizeof(float);
 		float* d_tmp = static_cast<float*>(h->d_tmp.reserve(bytes));
 		launch_tensor_convert_out(h->geom, h->d_tensor.get(), h->d_scale.as<const float>(), d_tmp, s);

@@ -113,7 +113,12 @@ inline uint32_t pad_ntx(uint32_t ntx)
 }
 
 // Bytes per complex sample of a storage kind.
+// Real (RF) samples, float32: the DefaultSTAProcessor path (library-internal
+// storage kind; the C ABI selects it with mista_create_rf).
+constexpr int kStorageReal = 3;
 inline size_t storage_bytes(int storage) { return storage == MISTA_STORAGE_F32 ? 8 : 4; }
+// F16/BF16 carry a per-channel scale folded into the pair weights.
+inline bool storage_scaled(int storage) { return storage == MISTA_STORAGE_F16 || storage == MISTA_STORAGE_BF16; }
 
 // ---- kernels launchers (defined in the .hip files) ----
 void launch_delay_table(const float* d_grid_xz, uint64_t npix, uint32_t num_elements, double pitch,
@@ -132,6 +137,15 @@ struct PrepState {
 };
 void prepare_signals(PrepState& st, const FrameGeom& g, const float* d_raw, const float* d_filter_poly,
 		uint32_t taps_per_phase, uint32_t filter_offset, void* d_tensor, float* d_scale, hipStream_t s);
+// RF preparation (DefaultSTAProcessor.h:136-164): FIR x 4 + DC removal into a
+// real float tensor [ntx][nrx][L].
+void prepare_signals_rf(const FrameGeom& g, const float* d_raw, const float* d_filter_poly, uint32_t taps_per_phase,
+		uint32_t filter_offset, float* d_tensor, hipStream_t s);
+// Interpolator4X.h:54-72 filter (81 taps), from its recipe.
+std::vector<double> rf4x_filter();
+// Row envelope (HilbertEnvelope::calculate on every row of an n1 x n2
+// matrix, in place).
+void envelope_rows(PrepState& st, float* d_data, uint32_t n1, uint32_t n2, hipStream_t s);
 
 // Device bytes held by the signal-preparation state (plans' buffers and an
 // estimate of the FFT library's work area).

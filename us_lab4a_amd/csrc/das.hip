@@ -32,6 +32,7 @@ namespace das {
 extern template void launch_tile_storage<MISTA_STORAGE_F32>(int, const TileArgs&, uint32_t, uint32_t, hipStream_t);
 extern template void launch_tile_storage<MISTA_STORAGE_F16>(int, const TileArgs&, uint32_t, uint32_t, hipStream_t);
 extern template void launch_tile_storage<MISTA_STORAGE_BF16>(int, const TileArgs&, uint32_t, uint32_t, hipStream_t);
+extern template void launch_tile_storage<kStorageReal>(int, const TileArgs&, uint32_t, uint32_t, hipStream_t);
 }  // namespace das
 
 using namespace das;
@@ -72,7 +73,7 @@ __global__ void __launch_bounds__(256) das_gather_kernel(const DasArgs a)
 				}
 			}
 		}
-		a.value[p] = sqrtf(sr * sr + si * si) * a.inv_num;
+		a.value[p] = a.signed_value ? sr * a.inv_num : sqrtf(sr * sr + si * si) * a.inv_num;
 		if (a.factor) a.factor[p] = 1.0f;
 	} else {
 		PixelStats<CF> st;
@@ -221,6 +222,7 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 		a.cf_factor = g.cf_param;
 	}
 	a.inv_num = 1.0f / static_cast<float>(static_cast<uint64_t>(g.ntx) * g.nrx);
+	a.signed_value = (g.storage == kStorageReal) ? 1 : 0;
 
 	// Tile kernel.  p0: tile shape, p1: chunking (below).
 	// window start entries are u32 (channel * L + ws): tensor below 2^32 entries
@@ -305,6 +307,7 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 			switch (g.storage) {
 			case MISTA_STORAGE_F32: launch_tile_storage<MISTA_STORAGE_F32>(g.cf_kind, A, ntiles, tc, s); break;
 			case MISTA_STORAGE_F16: launch_tile_storage<MISTA_STORAGE_F16>(g.cf_kind, A, ntiles, tc, s); break;
+			case kStorageReal: launch_tile_storage<kStorageReal>(g.cf_kind, A, ntiles, tc, s); break;
 			default: launch_tile_storage<MISTA_STORAGE_BF16>(g.cf_kind, A, ntiles, tc, s); break;
 			}
 			HIP_CHECK(hipGetLastError());
@@ -314,6 +317,7 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 	switch (g.storage) {
 	case MISTA_STORAGE_F32: launch_gather<MISTA_STORAGE_F32>(g, a, s); break;
 	case MISTA_STORAGE_F16: launch_gather<MISTA_STORAGE_F16>(g, a, s); break;
+	case kStorageReal: launch_gather<kStorageReal>(g, a, s); break;
 	default: launch_gather<MISTA_STORAGE_BF16>(g, a, s); break;
 	}
 	HIP_CHECK(hipGetLastError());

@@ -18,6 +18,8 @@ __device__ __forceinline__ float2 load_sample(const void* __restrict__ t, uint64
 		return static_cast<const float2*>(t)[i];
 	} else if constexpr (STORAGE == MISTA_STORAGE_F16) {
 		return __half22float2(static_cast<const __half2*>(t)[i]);
+	} else if constexpr (STORAGE == kStorageReal) {
+		return make_float2(static_cast<const float*>(t)[i], 0.0f);
 	} else {
 		const uint32_t b = static_cast<const uint32_t*>(t)[i];
 		return make_float2(__uint_as_float(b << 16), __uint_as_float(b & 0xFFFF0000u));
@@ -42,6 +44,11 @@ template<> struct Entry<MISTA_STORAGE_BF16> {
 	}
 };
 
+template<> struct Entry<kStorageReal> {
+	using T = float;
+	static __device__ __forceinline__ float2 get(T v) { return make_float2(v, 0.0f); }
+};
+
 struct DasArgs {
 	const void* tensor;   // [ntx][nrx][L] complex, storage type
 	const float* delay;   // [num_elements][npix]
@@ -54,6 +61,7 @@ struct DasArgs {
 	float offset;
 	float cf_factor;      // PCF: gamma / sigma0; SCF: p
 	float inv_num;        // 1 / (Ntx * Nrx)
+	int signed_value;     // RF samples: value = sum / (Ntx * Nrx), not |sum|
 };
 
 // Shifted single-pass moments for the two-pass population sigma of
@@ -97,7 +105,7 @@ struct PixelStats {
 	}
 	__device__ __forceinline__ void write(const DasArgs& a, uint64_t p) const
 	{
-		a.value[p] = sqrtf(sr * sr + si * si) * a.inv_num;
+		a.value[p] = a.signed_value ? sr * a.inv_num : sqrtf(sr * sr + si * si) * a.inv_num;
 		if (!a.factor) return;
 		float f = 1.0f;
 		if constexpr (CF == MISTA_CF_PCF) {

@@ -264,10 +264,13 @@ struct AnalyticArgs {
 	const float2* tw;      // Stockham twiddles exp(-2 pi i * i * j / (Ns * R))
 	const float2* wpost;   // exp(-2 pi i k / N), k in [0, M/2]
 	float2* scratch;       // [blocks][M]
-	float2* out;           // [blocks][L]
+	float2* out;           // [blocks][L] analytic signal, or
+	float* out_mag;        // [blocks][L] its magnitude (kOutMagnitude)
 	uint32_t ns, up, J, off, L, N, dz;
+	uint32_t flags;        // kRemoveDc | kOutMagnitude
 	FftPlanArgs plan;
 };
+constexpr uint32_t kRemoveDc = 1, kOutMagnitude = 2;
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
@@ -426,17 +429,19 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 		Df[i] = acc;
 	}
 	for (uint32_t i = L + tid; i < N; i += kFftThreads) Df[i] = 0.0f;
-	for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
-	if ((tid & 63) == 0) red[tid >> 6] = part;
-	__syncthreads();
-	if (tid == 0) {
-		double s = 0.0;
-		for (int w = 0; w < kFftThreads / 64; ++w) s += red[w];
-		red[0] = s;
+	if (A.flags & kRemoveDc) {
+		for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
+		if ((tid & 63) == 0) red[tid >> 6] = part;
+		__syncthreads();
+		if (tid == 0) {
+			double s = 0.0;
+			for (int w = 0; w < kFftThreads / 64; ++w) s += red[w];
+			red[0] = s;
+		}
+		__syncthreads();
+		const float mean = static_cast<float>(red[0] / static_cast<double>(L - A.dz));
+		for (uint32_t i = tid; i < L; i += kFftThreads) Df[i] = Df[i] - mean;
 	}
-	__syncthreads();
-	const float mean = static_cast<float>(red[0] / static_cast<double>(L - A.dz));
-	for (uint32_t i = tid; i < L; i += kFftThreads) Df[i] = Df[i] - mean;
 	__syncthreads();
 
 	fft_forward(D, A.plan, A.tw);
@@ -479,20 +484,22 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 	__syncthreads();
 
 	const float coef = 2.0f / static_cast<float>(N);
+	const bool mag = (A.flags & kOutMagnitude) != 0;
 	float2* dst = A.out + static_cast<uint64_t>(ch) * L;
+	float* dmag = A.out_mag + static_cast<uint64_t>(ch) * L;
+	// sample n of the analytic signal: (2/N) conj(transform output)
+	auto put = [&](uint32_t n, float2 v) {
+		const float re = v.x * coef, im = -v.y * coef;
+		if (mag) dmag[n] = sqrtf(re * re + im * im);
+		else dst[n] = make_float2(re, im);
+	};
 	fft_forward(D, A.plan, A.tw);
-	for (uint32_t m = tid; 2 * m < L; m += kFftThreads) {
-		const float2 v = D[m];
-		dst[2 * m] = make_float2(v.x * coef, -v.y * coef);
-	}
+	for (uint32_t m = tid; 2 * m < L; m += kFftThreads) put(2 * m, D[m]);
 	__syncthreads();
 	for (uint32_t m = tid; m < M; m += kFftThreads) D[m] = bo[m];
 	__syncthreads();
 	fft_forward(D, A.plan, A.tw);
-	for (uint32_t m = tid; 2 * m + 1 < L; m += kFftThreads) {
-		const float2 v = D[m];
-		dst[2 * m + 1] = make_float2(v.x * coef, -v.y * coef);
-	}
+	for (uint32_t m = tid; 2 * m + 1 < L; m += kFftThreads) put(2 * m + 1, D[m]);
 }
 
 // ---------------------------------------------------------------- storage
@@ -641,7 +648,7 @@ void launch_weights(const FrameGeom& g, const float* d_rx_apod, const float* d_t
 {
 	const uint32_t n = g.ntx_pad * g.nrx;
 	hipLaunchKernelGGL(weights_kernel, dim3((n + 255) / 256), dim3(256), 0, s, g.ntx, g.ntx_pad, g.nrx, d_rx_apod,
-			d_tx_apod, g.storage == MISTA_STORAGE_F32 ? nullptr : d_scale, d_w);
+			d_tx_apod, storage_scaled(g.storage) ? d_scale : nullptr, d_w);
 	HIP_CHECK(hipGetLastError());
 }
 
@@ -781,6 +788,8 @@ static void prepare_fused(PrepState& st, const FrameGeom& g, const float* d_raw,
 	A.L = g.L;
 	A.N = g.nf;
 	A.dz = g.dz_up;
+	A.flags = kRemoveDc;
+	A.out_mag = nullptr;
 	A.plan = P.fplan;
 	for (uint32_t c0 = 0; c0 < nch; c0 += chunk) {
 		const uint32_t n = std::min(chunk, nch - c0);
@@ -883,6 +892,89 @@ void prepare_signals(PrepState& st, const FrameGeom& g, const float* d_raw, cons
 					static_cast<uint64_t>(c0));
 			break;
 		}
+		HIP_CHECK(hipGetLastError());
+	}
+}
+
+// ---------------------------------------------------------------- RF path
+
+std::vector<double> rf4x_filter()
+{
+	// Interpolator4X.h:54-72: upFactor 4, nPeriods 10, beta 5:
+	// h = sinc(w) .* kaiser(81, 5), w = linspace(-10, 10, 81)
+	const unsigned n = 81;
+	const double beta = 5.0;
+	const double i0b = std::cyl_bessel_i(0.0, beta);
+	std::vector<double> h(n);
+	for (unsigned i = 0; i < n; ++i) {
+		const double w = -10.0 + 20.0 * i / (n - 1.0);
+		const double s = (w == 0.0) ? 1.0 : std::sin(M_PI * w) / (M_PI * w);
+		const double r = 2.0 * i / (n - 1.0) - 1.0;
+		double a = 1.0 - r * r;
+		if (a < 0) a = 0;
+		h[i] = s * std::cyl_bessel_i(0.0, beta * std::sqrt(a)) / i0b;
+	}
+	return h;
+}
+
+void prepare_signals_rf(const FrameGeom& g, const float* d_raw, const float* d_filter_poly, uint32_t J, uint32_t off,
+		float* d_tensor, hipStream_t s)
+{
+	const uint32_t nch = g.ntx * g.nrx;
+	const uint32_t chunk = std::min<uint32_t>(nch, 1024u);
+	const size_t lds = (static_cast<size_t>(J) + g.ns + off / g.up + 2 + static_cast<size_t>(g.up) * J) * sizeof(float);
+	if (lds > 150 * 1024) MISTA_THROW(MISTA_E_INVALID_PARAM, "channel too long for the LDS FIR: " << g.ns << " samples");
+	const uint32_t vpt = (g.L + kFirThreads - 1) / kFirThreads;
+	if (vpt > 64) MISTA_THROW(MISTA_E_INVALID_PARAM, "signal length " << g.L << " exceeds 65536");
+	FrameGeom gr = g;
+	gr.nf = g.L;   // rows written back to back, no FFT padding
+	for (uint32_t c0 = 0; c0 < nch; c0 += chunk) {
+		const uint32_t n = std::min(chunk, nch - c0);
+		const float* raw = d_raw + static_cast<uint64_t>(c0) * g.ns;
+		float* out = d_tensor + static_cast<uint64_t>(c0) * g.L;
+		if (vpt <= 4) launch_fir<4>(n, gr, raw, d_filter_poly, J, off, out, s);
+		else if (vpt <= 8) launch_fir<8>(n, gr, raw, d_filter_poly, J, off, out, s);
+		else if (vpt <= 16) launch_fir<16>(n, gr, raw, d_filter_poly, J, off, out, s);
+		else if (vpt <= 32) launch_fir<32>(n, gr, raw, d_filter_poly, J, off, out, s);
+		else launch_fir<64>(n, gr, raw, d_filter_poly, J, off, out, s);
+		HIP_CHECK(hipGetLastError());
+	}
+}
+
+void envelope_rows(PrepState& st, float* d_data, uint32_t n1, uint32_t n2, hipStream_t s)
+{
+	if (n1 == 0 || n2 == 0) return;
+	FrameGeom g{};
+	g.ns = n2;
+	g.up = 1;
+	g.L = n2;
+	g.nf = hilbert_fft_size(n2);
+	const size_t lds = fused_lds_bytes(g, 0, 0);
+	if (!lds) MISTA_THROW(MISTA_E_INVALID_PARAM, "rows of " << n2 << " points are too long for the on-chip envelope");
+	if (!st.plan) st.plan = new PrepPlan();
+	PrepPlan& P = *st.plan;
+	if (P.fused_nf != g.nf) build_fused_plan(P, g.nf, s);
+	const uint32_t M = g.nf / 2;
+	const uint32_t chunk = std::min<uint32_t>(n1, 1024u);
+	float2* scratch = static_cast<float2*>(P.spec.reserve(static_cast<size_t>(chunk) * M * sizeof(float2)));
+	HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(analytic_lds_kernel),
+			hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+	AnalyticArgs A{};
+	A.tw = P.tw.as<const float2>();
+	A.wpost = P.wpost.as<const float2>();
+	A.scratch = scratch;
+	A.ns = n2;
+	A.up = 1;
+	A.L = n2;
+	A.N = g.nf;
+	A.flags = kOutMagnitude;
+	A.plan = P.fplan;
+	for (uint32_t r0 = 0; r0 < n1; r0 += chunk) {
+		const uint32_t n = std::min(chunk, n1 - r0);
+		A.raw = d_data + static_cast<uint64_t>(r0) * n2;
+		A.out = nullptr;
+		A.out_mag = d_data + static_cast<uint64_t>(r0) * n2;   // in place: a row is in LDS before it is written
+		hipLaunchKernelGGL(analytic_lds_kernel, dim3(n), dim3(kFftThreads), lds, s, A);
 		HIP_CHECK(hipGetLastError());
 	}
 }

@@ -343,3 +343,56 @@ class HipSTA3DProcessor(HipSTAProcessor):
         out = np.empty((self.config.num_rx + self.config.num_tx, n1 * n2), np.float32)
         check(lib().mista_download_delays(self._h, out.ctypes.data), self._h)
         return out
+
+
+class HipSTARFProcessor(HipSTAProcessor):
+    """DefaultSTAProcessor on MI355X (DefaultSTAProcessor.h:50-226): real
+    samples upsampled by 4 with the fixed 81-tap filter, no apodization;
+    process() writes value = signed sum / (Ntx*Nrx) and factor = SCF (or 1).
+    envelope() is the row envelope STAMethod takes afterwards."""
+
+    def __init__(self, config, acquisition, coherence_factor, peak_offset, device=0):
+        self.config = config
+        self.acquisition = acquisition
+        self.coherence_factor = coherence_factor if coherence_factor is not None else CoherenceFactor()
+        c = _lib.StaConfig()
+        c.num_elements = config.num_elements
+        c.first_tx_elem = config.first_tx_elem
+        c.last_tx_elem = config.last_tx_elem
+        c.upsampling_factor = 4
+        c.pitch = config.pitch
+        c.sampling_frequency = config.sampling_frequency
+        c.propagation_speed = config.propagation_speed
+        c.center_frequency = config.center_frequency
+        c.dead_zone_m = config.dead_zone_m
+        c.peak_offset = peak_offset
+        c.cf_kind = self.coherence_factor.kind
+        c.cf_param = self.coherence_factor.param
+        c.storage = _lib.STORAGE_F32
+        c.device = device
+        h = ctypes.c_void_p()
+        check(lib().mista_create_rf(ctypes.byref(c), ctypes.byref(h)))
+        self._h = h
+        self.upsampling_factor = 4
+        self.storage = "rf"
+        self._grid_shape = None
+
+    def upload_analytic(self, tensor):
+        """Real tensor [ntx][nrx][L] (the prepared RF signals)."""
+        a = np.ascontiguousarray(tensor, dtype=np.float32)
+        check(lib().mista_upload_analytic(self._h, a.ctypes.data, a.shape[2]), self._h)
+
+    def download_analytic(self):
+        L = self.query()[0]
+        out = np.empty((self.config.num_tx, self.config.num_elements, L), np.float32)
+        check(lib().mista_download_analytic(self._h, out.ctypes.data), self._h)
+        return out
+
+    def envelope(self, values):
+        """Row envelope of a host float32 [n1][n2] image (returns a copy)."""
+        v = np.array(values, dtype=np.float32, order="C", copy=True)
+        check(lib().mista_envelope(self._h, v.ctypes.data, v.shape[0], v.shape[1]), self._h)
+        return v
+
+    def envelope_device(self, d_values, n1, n2, stream=None):
+        check(lib().mista_envelope_device(self._h, _ptr(d_values), n1, n2, _stream_ptr(stream)), self._h)
