@@ -47,3 +47,54 @@ def test_cli_matches_oracle(tmp_path, cf):
         cfimg = np.fromfile(str(out) + "_cf.f32", np.float32).reshape(wl.n_x, wl.n_z)
         np.testing.assert_array_equal(cfimg, v * f)
         assert float(np.abs(v * f - v_o * f_o).max()) / peak <= 1e-4
+
+
+H5DUMP = "/opt/conda/bin/h5dump"
+
+
+@pytest.mark.skipif(not os.path.exists(H5DUMP), reason="h5dump absent")
+def test_cli_hdf5_acquisition_and_images(tmp_path):
+    """Saved acquisition in the reference's HDF5 layout in, the reference's
+    image files out (image_value/factor/cf/x/z.h5): same images as the raw
+    path, within 1e-4 of the oracle."""
+    wl = workloads.sta2d_32_small()
+    wl.tx_apod = None
+    echoes = wl.echoes_cpu()
+    raw = tmp_path / "e.f32"
+    echoes.astype(np.float32).tofile(raw)
+    acq = tmp_path / "acq"
+    acq.mkdir()
+    r = subprocess.run([CLI, "--echoes", str(raw), "--elements", str(wl.num_elements), "--samples",
+                        str(wl.num_samples), "--write-acq", str(acq)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    common = ["--elements", str(wl.num_elements), "--up", str(wl.upsampling), "--pitch", repr(wl.pitch),
+              "--fs", repr(wl.fs), "--c", repr(wl.c), "--fc", repr(wl.fc), "--grid", str(wl.n_x), str(wl.n_z),
+              repr(wl.grid_x[0]), repr(wl.grid_x[1]), repr(wl.grid_z[0]), repr(wl.grid_z[1]), "--cf", "scf"]
+    h5o = tmp_path / "img"
+    h5o.mkdir()
+    print("[gpu-test] cli hdf5", flush=True)
+    r = subprocess.run([CLI, "--acq-dir", str(acq)] + common + ["--out", str(tmp_path / "a"), "--h5-out", str(h5o)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([CLI, "--echoes", str(raw), "--samples", str(wl.num_samples)] + common +
+                       ["--out", str(tmp_path / "b")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+    def h5(name, ds):
+        b = tmp_path / f"{name}.bin"
+        subprocess.run([H5DUMP, "-d", ds, "-b", "LE", "-o", str(b), str(h5o / f"{name}.h5")], capture_output=True,
+                       timeout=60)
+        return np.fromfile(b, np.float64).reshape(wl.n_x, wl.n_z)
+
+    v_raw = np.fromfile(str(tmp_path / "b") + "_value.f32", np.float32).reshape(wl.n_x, wl.n_z)
+    f_raw = np.fromfile(str(tmp_path / "b") + "_factor.f32", np.float32).reshape(wl.n_x, wl.n_z)
+    np.testing.assert_array_equal(h5("image_value", "value"), v_raw.astype(np.float64))
+    np.testing.assert_array_equal(h5("image_factor", "factor"), f_raw.astype(np.float64))
+    np.testing.assert_array_equal(h5("image_cf", "cf"), (v_raw * f_raw).astype(np.float64))
+    g = imaging.rectangular_grid(wl.grid_x[0], wl.grid_x[1], wl.n_x, wl.grid_z[0], wl.grid_z[1], wl.n_z)
+    np.testing.assert_array_equal(h5("image_x", "x"), g["x"].astype(np.float64))
+    np.testing.assert_array_equal(h5("image_z", "z"), g["z"].astype(np.float64))
+    cfg = wl.oracle_cfg()
+    cfg.cf_kind = oracle.CF_SCF
+    v_o, _, _, _ = oracle.process(cfg, echoes, imaging.grid_xz(g), wl.rx_apodization())
+    assert float(np.abs(v_raw - v_o).max()) / float(np.abs(v_o).max()) <= 1e-4

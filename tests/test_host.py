@@ -179,3 +179,32 @@ def test_workload_sizes():
     assert oracle.next_fast_even(32768 + 1024) == 34560
     sp = synth.speckle_scatterers(3, 5000.0, w2.grid_x, w2.grid_z)
     assert len(sp) == 114000
+
+
+H5DUMP = "/opt/conda/bin/h5dump"
+CLI = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "us_lab4a_amd", "mista_sta_cli")
+
+
+@pytest.mark.skipif(not (os.path.exists(H5DUMP) and os.path.exists(CLI)), reason="h5dump or the host driver absent")
+def test_cli_writes_reference_acquisition_layout(tmp_path):
+    """--write-acq stores one HDF5 file per transmit in the reference's
+    saved-acquisition layout (FileUtil.cpp txElemSignalsPath, dataset
+    "signal" [rx][samples], IEEE_F64LE as HDF5Util::save writes)."""
+    import subprocess
+    rng = np.random.default_rng(2)
+    echoes = rng.standard_normal((2, 3, 5)).astype(np.float32)
+    raw = tmp_path / "e.f32"
+    echoes.tofile(raw)
+    out = tmp_path / "acq"
+    out.mkdir()
+    r = subprocess.run([CLI, "--echoes", str(raw), "--elements", "3", "--samples", "5", "--write-acq", str(out),
+                        "--base", "7"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    for t in range(2):
+        f = out / f"signals-base0007-tx{t:04d}.h5"
+        assert f.exists()
+        head = subprocess.run([H5DUMP, "-H", str(f)], capture_output=True, text=True, timeout=60).stdout
+        assert 'DATASET "signal"' in head and "H5T_IEEE_F64LE" in head and "( 3, 5 )" in head
+        b = tmp_path / f"t{t}.bin"
+        subprocess.run([H5DUMP, "-d", "signal", "-b", "LE", "-o", str(b), str(f)], capture_output=True, timeout=60)
+        np.testing.assert_array_equal(np.fromfile(b, np.float64).reshape(3, 5), echoes[t].astype(np.float64))

@@ -74,9 +74,22 @@ def build_host_driver(verbose=True):
         return None
     if verbose:
         print("[build]", os.path.relpath(src, ROOT), flush=True)
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
-          "-I", os.path.join(PKG, "include"), src, "-o", HOST_DRIVER,
-          "-L", PKG, "-lmista_hip", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN"])
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(PKG, "include"), src, "-o", HOST_DRIVER,
+           "-L", PKG, "-lmista_hip", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN"]
+    # HDF5 for the reference's acquisition/image files.  The conda HDF5 is
+    # reached through a private directory holding only its library, so the
+    # driver never picks up conda's other runtime libraries.
+    h5_inc, h5_lib = "/opt/conda/include/hdf5.h", "/opt/conda/lib/libhdf5.so.103"
+    if os.path.exists(h5_inc) and os.path.exists(h5_lib):
+        libdir = os.path.join(PKG, "host", "lib")
+        os.makedirs(libdir, exist_ok=True)
+        link = os.path.join(libdir, "libhdf5.so.103")
+        if os.path.islink(link) or os.path.exists(link):
+            os.remove(link)
+        os.symlink(h5_lib, link)
+        cmd += ["-DMISTA_HAVE_HDF5", "-I", os.path.dirname(h5_inc), link, "-Wl,-rpath,$ORIGIN/host/lib"]
+    _run(cmd)
     return HOST_DRIVER
 
 

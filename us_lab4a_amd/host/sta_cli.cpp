@@ -7,14 +7,20 @@
 // Echoes come from a raw float32 file [ntx][nrx][ns] (what the reference's
 // SavedSTAAcquisition replays per transmit, SavedSTAAcquisition.h:78-90).
 //
-//   mista_sta_cli --echoes FILE --elements N --samples NS --up U
-//                 --pitch M --fs HZ --c MPS --fc HZ
+//   mista_sta_cli (--echoes FILE | --acq-dir DIR [--base B]) --elements N
+//                 --samples NS --up U --pitch M --fs HZ --c MPS --fc HZ
 //                 --grid N1 N2 XMIN XMAX ZMIN ZMAX
 //                 [--cf none|pcf|scf] [--cf-param P] [--apod hanning|rectangular]
-//                 [--storage f32|f16|bf16] [--out PREFIX]
+//                 [--storage f32|f16|bf16] [--out PREFIX] [--h5-out DIR]
+//   mista_sta_cli --echoes FILE --elements N --samples NS --write-acq DIR [--base B]
 //
-// Writes PREFIX_value.f32 and PREFIX_factor.f32 ([N1][N2], z fastest) and,
-// with a coherence factor, PREFIX_cf.f32 (value * factor).
+// Echoes come from a raw float32 file [tx][rx][ns] or from the reference's
+// saved acquisition (DIR/signals-base%04d-tx%04d.h5, dataset "signal"
+// [rx][samples]; transmits 0 .. last present).  Writes PREFIX_value.f32 and
+// PREFIX_factor.f32 ([N1][N2], z fastest) and, with a coherence factor,
+// PREFIX_cf.f32 (value * factor); with --h5-out, the reference's image files
+// image_value/factor/cf/x/z.h5.  --write-acq converts a raw echo file into
+// the saved-acquisition layout and exits (no GPU use).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -26,6 +32,9 @@
 #include <vector>
 
 #include "mista/HipSTAProcessor.h"
+#ifdef MISTA_HAVE_HDF5
+#include "h5io.h"
+#endif
 
 namespace {
 
@@ -78,7 +87,8 @@ bool write_f32(const std::string& path, const std::vector<float>& v)
 int main(int argc, char** argv)
 {
 	std::string echoes_path, out = "sta", cf = "none", apod = "hanning", storage = "f32";
-	unsigned int N = 0, ns = 0, up = 1, n1 = 0, n2 = 0;
+	std::string acq_dir, h5_out, write_acq;
+	unsigned int N = 0, ns = 0, up = 1, n1 = 0, n2 = 0, base = 0;
 	float pitch = 0, fs = 0, c = 1540.0f, fc = 0, cf_param = 1.0f;
 	double xmin = 0, xmax = 0, zmin = 0, zmax = 0;
 	for (int i = 1; i < argc; ++i) {
@@ -100,25 +110,69 @@ int main(int argc, char** argv)
 		else if (a == "--apod") apod = next();
 		else if (a == "--storage") storage = next();
 		else if (a == "--out") out = next();
+		else if (a == "--acq-dir") acq_dir = next();
+		else if (a == "--base") base = std::atoi(next());
+		else if (a == "--h5-out") h5_out = next();
+		else if (a == "--write-acq") write_acq = next();
 		else if (a == "--grid") {
 			n1 = std::atoi(next()); n2 = std::atoi(next());
 			xmin = std::atof(next()); xmax = std::atof(next()); zmin = std::atof(next()); zmax = std::atof(next());
 		} else { std::cerr << "unknown argument " << a << "\n"; return 2; }
 	}
-	if (echoes_path.empty() || N == 0 || ns == 0 || n1 == 0 || n2 == 0) {
+#ifndef MISTA_HAVE_HDF5
+	if (!acq_dir.empty() || !h5_out.empty() || !write_acq.empty()) {
+		std::cerr << "built without HDF5 support\n";
+		return 2;
+	}
+#endif
+	const bool convert = !write_acq.empty();
+	if ((echoes_path.empty() && acq_dir.empty()) || N == 0 || (ns == 0 && acq_dir.empty()) ||
+			(!convert && (n1 == 0 || n2 == 0))) {
 		std::cerr << "usage: see the header of sta_cli.cpp\n";
 		return 2;
 	}
 	try {
-		std::ifstream f(echoes_path, std::ios::binary | std::ios::ate);
-		if (!f) throw mista::InvalidParameterException("cannot open " + echoes_path);
-		const size_t bytes = static_cast<size_t>(f.tellg());
-		const size_t per_tx = static_cast<size_t>(N) * ns * sizeof(float);
-		if (bytes % per_tx != 0) throw mista::InvalidParameterException("echo file size is not a multiple of one transmit");
-		std::vector<float> data(bytes / sizeof(float));
-		f.seekg(0);
-		f.read(reinterpret_cast<char*>(data.data()), static_cast<std::streamsize>(bytes));
-		const unsigned int ntx = static_cast<unsigned int>(bytes / per_tx);
+		std::vector<float> data;
+		unsigned int ntx = 0;
+		if (!echoes_path.empty()) {
+			std::ifstream f(echoes_path, std::ios::binary | std::ios::ate);
+			if (!f) throw mista::InvalidParameterException("cannot open " + echoes_path);
+			const size_t bytes = static_cast<size_t>(f.tellg());
+			const size_t per_tx = static_cast<size_t>(N) * ns * sizeof(float);
+			if (bytes % per_tx != 0) throw mista::InvalidParameterException("echo file size is not a multiple of one transmit");
+			data.resize(bytes / sizeof(float));
+			f.seekg(0);
+			f.read(reinterpret_cast<char*>(data.data()), static_cast<std::streamsize>(bytes));
+			ntx = static_cast<unsigned int>(bytes / per_tx);
+		}
+#ifdef MISTA_HAVE_HDF5
+		if (convert) {
+			// Project::saveTxElemSignalsToHDF5 layout, one file per transmit
+			for (unsigned int t = 0; t < ntx; ++t)
+				mista_h5::write2d(mista_h5::tx_signals_path(write_acq, base, t), "signal",
+						data.data() + static_cast<size_t>(t) * N * ns, N, ns);
+			std::printf("wrote %u transmits to %s\n", ntx, write_acq.c_str());
+			return 0;
+		}
+		if (!acq_dir.empty()) {
+			// SavedSTAAcquisition::execute for transmits 0, 1, ... while files exist
+			for (unsigned int t = 0;; ++t) {
+				const std::string path = mista_h5::tx_signals_path(acq_dir, base, t);
+				if (std::FILE* probe = std::fopen(path.c_str(), "rb")) std::fclose(probe);
+				else break;
+				size_t r = 0, c = 0;
+				std::vector<float> sig = mista_h5::read2d(path, "signal", r, c);
+				if (r != N)
+					throw mista::InvalidValueException("Invalid number of rx elements (" + std::to_string(r) + ") in file " +
+							path + ", should be " + std::to_string(N) + ".");
+				if (t == 0) ns = static_cast<unsigned int>(c);
+				else if (c != ns) throw mista::InvalidValueException("The number of samples changed between transmits.");
+				data.insert(data.end(), sig.begin(), sig.end());
+				ntx = t + 1;
+			}
+			if (ntx == 0) throw mista::InvalidParameterException("no acquisition files in " + acq_dir);
+		}
+#endif
 
 		mista::STAConfiguration config;
 		config.numElements = N;
@@ -162,6 +216,23 @@ int main(int argc, char** argv)
 			std::cerr << "cannot write outputs\n";
 			return 1;
 		}
+#ifdef MISTA_HAVE_HDF5
+		if (!h5_out.empty()) {
+			// Project::saveImageToHDF5 / saveXYZToHDF5 / saveFactorToHDF5
+			std::vector<float> xs(grid.pts.size()), zs(grid.pts.size());
+			for (size_t i = 0; i < grid.pts.size(); ++i) {
+				xs[i] = grid.pts[i].x;
+				zs[i] = grid.pts[i].z;
+			}
+			mista_h5::write2d(h5_out + "/image_value.h5", "value", value.data(), n1, n2);
+			mista_h5::write2d(h5_out + "/image_x.h5", "x", xs.data(), n1, n2);
+			mista_h5::write2d(h5_out + "/image_z.h5", "z", zs.data(), n1, n2);
+			if (coherence.enabled()) {
+				mista_h5::write2d(h5_out + "/image_factor.h5", "factor", factor.data(), n1, n2);
+				mista_h5::write2d(h5_out + "/image_cf.h5", "cf", cfimg.data(), n1, n2);
+			}
+		}
+#endif
 		std::printf("frame: %u tx x %u rx x %u samples, grid %u x %u, %.3f ms (host path incl. transfers)\n", ntx, N, ns, n1,
 				n2, std::chrono::duration<double, std::milli>(t1 - t0).count());
 	} catch (const std::exception& e) {
