@@ -315,7 +315,8 @@ def main():
         "data": "synthetic (GPU-generated seeded point + speckle scatterers, AWGN 30 dB)",
         "config": {"workload": wl.name, "elements": wl.num_elements, "tx": wl.num_tx, "samples": wl.num_samples,
                    "upsampling": wl.upsampling, "grid": [wl.n_x, wl.n_z], "coherence": wl.cf,
-                   "storage": args.storage, "pixel_pairs_per_frame": wl.pixel_pairs},
+                   "storage": args.storage, "pixel_pairs_per_frame": wl.pixel_pairs,
+                   "das_tile": tile_name(proc.tuned_kernel())},
         "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                              "p99": round(float(np.percentile(lat, 99)), 4)},
         "stage_ms": {"prep": round(prep_avg, 4), "das": round(das_avg, 4)},
@@ -333,6 +334,15 @@ def main():
         out["speedup_vs_cpu"] = round(value / cb["value"], 1)
     proc.close()
     print(json.dumps(out), flush=True)
+
+
+def tile_name(tuned):
+    """Human-readable tile choice of the DAS autotuner."""
+    p0, p1 = tuned
+    if p0 < 0:
+        return "gather"
+    shape = {2: "8x32", 3: "32x8", 4: "8x8x4"}.get(p0, "16x16")
+    return f"{shape} tile, {(p1 >> 16) & 0xFF} KiB LDS ring"
 
 
 if __name__ == "__main__":

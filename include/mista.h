@@ -201,6 +201,12 @@ int mista_query(const mista_handle* h, uint32_t* signal_length, uint32_t* fft_si
  * one-pixel-per-lane gather, 2 = LDS-tiled).  Tile parameters 0 = default. */
 int mista_set_kernel(mista_handle* h, int variant, int tile_param0, int tile_param1);
 
+/* Tile parameters the automatic variant chose on its first frame of the
+ * current geometry (it times each tile shape and LDS budget once); -1 in
+ * tile_param0 when nothing was tuned or the gather kernel runs.  No
+ * counterpart in the reference (its kernels have fixed launch shapes). */
+int mista_tuned_kernel(const mista_handle* h, int* tile_param0, int* tile_param1);
+
 /*
  * RF processor: replaces DefaultSTAProcessor (DefaultSTAProcessor.h:50-226;
  * constructor (config, acquisition, coherenceFactor, peakOffset)).  Echoes are

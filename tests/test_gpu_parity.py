@@ -265,6 +265,34 @@ def test_tiled_kernel_equals_gather_kernel(small, tz, storage):
     wl.cf = "none"
 
 
+def test_autotuned_kernel_equals_gather_kernel(small):
+    """The automatic variant times every tile shape and LDS budget on its
+    first frame and keeps the fastest: that frame and the tuned frames after
+    it are bit-identical to the gather kernel's."""
+    import torch
+    wl, echoes = small
+    wl.cf = "phase_coherence_factor"
+    p = make_proc(wl)
+    p.process_echoes(echoes)
+    n = wl.n_x * wl.n_z
+    outs = []
+    for variant in (0, 0, 1):
+        p.set_kernel(variant)
+        dv = torch.empty(n, dtype=torch.float32, device="cuda")
+        df = torch.empty(n, dtype=torch.float32, device="cuda")
+        p.das_device(dv, df)
+        torch.cuda.synchronize()
+        outs.append((dv.cpu().numpy(), df.cpu().numpy()))
+        if variant == 0:
+            tuned = p.tuned_kernel()
+            say("autotuner chose", tuned)
+            assert tuned[0] in (1, 2, 3) and (tuned[1] >> 16) in (52, 76)
+    for v, f in outs[:2]:
+        assert np.array_equal(v, outs[2][0]) and np.array_equal(f, outs[2][1])
+    p.close()
+    wl.cf = "none"
+
+
 def test_error_paths_raise_reference_style_errors():
     wl = workloads.tiny()
     say("error paths")

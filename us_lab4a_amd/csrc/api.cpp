@@ -732,6 +732,14 @@ int mista_set_kernel(mista_handle* h, int variant, int p0, int p1)
 	});
 }
 
+int mista_tuned_kernel(const mista_handle* h, int* p0, int* p1)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	if (p0) *p0 = h->das_scratch.tuned_p0;
+	if (p1) *p1 = h->das_scratch.tuned_p1;
+	return MISTA_OK;
+}
+
 int mista_set_prep(mista_handle* h, int method)
 {
 	if (!h) return MISTA_E_INVALID_PARAM;

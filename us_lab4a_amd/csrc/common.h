@@ -167,6 +167,8 @@ struct DasScratch {
 	float params_offset = 0.0f;
 	uint32_t wcap = 0;              // LDS window entries per staged pair
 	uint32_t max_slots = 0;
+	uint64_t tuned_key = ~0ull;     // geometry the autotuner last chose for
+	int tuned_p0 = -1, tuned_p1 = 0;
 };
 // Bytes of zeroed padding after the analytic tensor (window over-read).
 size_t tensor_tail_padding();

@@ -193,6 +193,97 @@ __global__ void __launch_bounds__(256) window_params_kernel(const float2* __rest
 
 }  // namespace
 
+// Builds (or reuses) the window tables for tile shape p0 and launches the
+// tiled kernel with the chunking p1.  p0: 0/1 = 16x16 (planes) or 8x8x4
+// (volumes), 2 = 8x32, 3 = 32x8, 4 = 8x8x4.  p1: byte 0 pairs per chunk
+// (4 or 8), byte 1 chunk buffers, byte 2 LDS budget in KiB (default: the
+// smallest of 52 (three workgroups per CU: the kernel is compiled for three
+// waves per SIMD), 76 (two) and 150 (one) in which the windows fit).
+// Returns false when the tables do not fit the tiled kernel.
+bool launch_tiled(const FrameGeom& g, int p0, int p1, DasScratch& scratch, const DasArgs& a, const float* d_delay,
+		hipStream_t s)
+{
+	const uint32_t ny = g.ny ? g.ny : 1u;
+	uint32_t tw1 = 16, tw2 = 1, tzh = 16;
+	if (ny > 1) { tw1 = 8; tw2 = 8; tzh = 4; }
+	if (p0 == 2) { tw1 = 8; tw2 = 1; tzh = 32; }
+	else if (p0 == 3) { tw1 = 32; tw2 = 1; tzh = 8; }
+	else if (p0 == 4) { tw1 = 8; tw2 = 8; tzh = 4; }
+	const uint32_t esz = static_cast<uint32_t>(storage_bytes(g.storage));
+	const uint32_t slot = 64 * (16 / esz);
+	const uint32_t nx = g.n1 / ny;
+	TileGeom tg{g.n1, g.n2, ny, nx, tw1, tw2, tzh, (nx + tw1 - 1) / tw1, (ny + tw2 - 1) / tw2, (g.n2 + tzh - 1) / tzh};
+	const uint64_t ntiles64 = static_cast<uint64_t>(tg.tiles_x) * tg.tiles_y * tg.tiles_z;
+	if (ntiles64 >= (1ull << 31)) return false;
+	const uint32_t ntiles = static_cast<uint32_t>(ntiles64);
+	const uint64_t key = (static_cast<uint64_t>(tw1) << 56) ^ (static_cast<uint64_t>(tw2) << 52) ^
+			(static_cast<uint64_t>(tzh) << 44) ^ (static_cast<uint64_t>(ny) << 40) ^
+			(static_cast<uint64_t>(esz) << 36) ^ (static_cast<uint64_t>(g.L) << 8) ^ g.first_tx ^
+			(static_cast<uint64_t>(g.ntx) << 20) ^ (static_cast<uint64_t>(g.ntx_pad) << 28);
+	if (scratch.bounds_version != g.grid_version || scratch.params_key != key || scratch.params_offset != g.offset) {
+		float2* b = static_cast<float2*>(scratch.bounds.reserve(static_cast<size_t>(ntiles) * g.num_elements * sizeof(float2)));
+		hipLaunchKernelGGL(tile_bounds_kernel, dim3(ntiles), dim3(256), 0, s, d_delay, g.npix, g.num_elements, tg, b);
+		HIP_CHECK(hipGetLastError());
+		const uint64_t np = static_cast<uint64_t>(ntiles) * g.nrx * g.ntx_pad;
+		WinParam* prm = static_cast<WinParam*>(scratch.params.reserve(np * sizeof(WinParam)));
+		uint32_t* d_max = static_cast<uint32_t*>(scratch.partial.reserve(256));
+		HIP_CHECK(hipMemsetAsync(d_max, 0, sizeof(uint32_t), s));
+		hipLaunchKernelGGL(window_params_kernel, dim3(static_cast<uint32_t>((np + 255) / 256)), dim3(256), 0, s, b, ntiles,
+				g.num_elements, g.first_tx, g.ntx, g.ntx_pad, g.nrx, g.L, g.offset, slot, prm, d_max);
+		HIP_CHECK(hipGetLastError());
+		uint32_t max_slots = 0;
+		HIP_CHECK(hipMemcpyAsync(&max_slots, d_max, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		scratch.wcap = std::max(1u, max_slots) * slot;
+		scratch.max_slots = max_slots;
+		scratch.bounds_version = g.grid_version;
+		scratch.params_key = key;
+		scratch.params_offset = g.offset;
+	}
+	if (scratch.max_slots > 511) return false;
+	// Pairs per chunk (tc) and chunk buffers in the LDS ring (nbuf): the most
+	// pairs in flight, (nbuf - 1) * tc, that fit the budget and the 6-bit
+	// vmcnt of the per-chunk waits.
+	const uint32_t S = scratch.wcap / slot;
+	const uint32_t want_tc = static_cast<uint32_t>(p1) & 0xFF, want_nb = (static_cast<uint32_t>(p1) >> 8) & 0xFF;
+	const uint32_t want_kb = (static_cast<uint32_t>(p1) >> 16) & 0xFF;
+	uint32_t tc = 0, nbuf = 0;
+	for (uint32_t kb : {52u, 76u, 150u}) {
+		if (want_kb && kb != 52u) break;
+		const size_t budget = static_cast<size_t>(want_kb ? std::min(want_kb, 158u) : kb) * 1024;
+		auto fit = [&](uint32_t c) -> uint32_t {
+			const size_t bb = static_cast<size_t>(c) * scratch.wcap * esz + kTileThreads * 4;
+			uint32_t nb = static_cast<uint32_t>(std::min<size_t>(16, budget / bb));
+			if (want_nb) nb = std::min(nb, want_nb);
+			const uint32_t per = c * S / (kTileThreads / 64) + 1;   // copies per wave per chunk, at most
+			while (nb > 2 && (nb - 2) * per > 63) --nb;
+			return (nb >= 2 && 2 * per <= 63) ? nb : 0u;
+		};
+		for (uint32_t c : {8u, 4u}) {
+			if (want_tc && c != want_tc) continue;
+			const uint32_t nb = fit(c);
+			if (nb && (nb - 1) * c > (nbuf ? (nbuf - 1) * tc : 0u)) { tc = c; nbuf = nb; }
+		}
+		if (nbuf) break;
+	}
+	if (nbuf == 0) return false;
+	TileArgs A;
+	A.d = a;
+	A.tg = tg;
+	A.params = scratch.params.as<const WinParam>();
+	A.wcap = scratch.wcap;
+	A.ntx_pad = g.ntx_pad;
+	A.nbuf = nbuf;
+	switch (g.storage) {
+	case MISTA_STORAGE_F32: launch_tile_storage<MISTA_STORAGE_F32>(g.cf_kind, A, ntiles, tc, s); break;
+	case MISTA_STORAGE_F16: launch_tile_storage<MISTA_STORAGE_F16>(g.cf_kind, A, ntiles, tc, s); break;
+	case kStorageReal: launch_tile_storage<kStorageReal>(g.cf_kind, A, ntiles, tc, s); break;
+	default: launch_tile_storage<MISTA_STORAGE_BF16>(g.cf_kind, A, ntiles, tc, s); break;
+	}
+	HIP_CHECK(hipGetLastError());
+	return true;
+}
+
 // Tail padding of the analytic tensor (bytes): staged windows may read up to
 // one load slot past the last sample of the last channel.
 size_t tensor_tail_padding() { return 64 * 16; }
@@ -224,94 +315,54 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 	a.inv_num = 1.0f / static_cast<float>(static_cast<uint64_t>(g.ntx) * g.nrx);
 	a.signed_value = (g.storage == kStorageReal) ? 1 : 0;
 
-	// Tile kernel.  p0: tile shape, p1: chunking (below).
 	// window start entries are u32 (channel * L + ws): tensor below 2^32 entries
-	bool tiled = (t.variant != 1) && g.ntx_pad <= 128 && g.L < (1u << 21) - 64 &&
+	const bool can_tile = (t.variant != 1) && g.ntx_pad <= 128 && g.L < (1u << 21) - 64 &&
 			static_cast<uint64_t>(g.ntx) * g.nrx * g.L < (1ull << 32) - (1ull << 22);
-	if (tiled) {
-		// 256 lanes per tile.  Planes: 16x16 (x, z); volumes: 8x8x4 (x, y, z).
-		// p0 overrides: 2 = 8x32, 3 = 32x8, 4 = 8x8x4.
-		const uint32_t ny = g.ny ? g.ny : 1u;
-		uint32_t tw1 = 16, tw2 = 1, tzh = 16;
-		if (ny > 1) { tw1 = 8; tw2 = 8; tzh = 4; }
-		if (t.p0 == 2) { tw1 = 8; tw2 = 1; tzh = 32; }
-		else if (t.p0 == 3) { tw1 = 32; tw2 = 1; tzh = 8; }
-		else if (t.p0 == 4) { tw1 = 8; tw2 = 8; tzh = 4; }
-		const uint32_t esz = static_cast<uint32_t>(storage_bytes(g.storage));
-		const uint32_t slot = 64 * (16 / esz);
-		const uint32_t nx = g.n1 / ny;
-		TileGeom tg{g.n1, g.n2, ny, nx, tw1, tw2, tzh, (nx + tw1 - 1) / tw1, (ny + tw2 - 1) / tw2, (g.n2 + tzh - 1) / tzh};
-		const uint64_t ntiles64 = static_cast<uint64_t>(tg.tiles_x) * tg.tiles_y * tg.tiles_z;
-		if (ntiles64 >= (1ull << 31)) tiled = false;
-		const uint32_t ntiles = static_cast<uint32_t>(ntiles64);
-		const uint64_t key = (static_cast<uint64_t>(tw1) << 56) ^ (static_cast<uint64_t>(tw2) << 52) ^
-				(static_cast<uint64_t>(tzh) << 44) ^ (static_cast<uint64_t>(ny) << 40) ^
-				(static_cast<uint64_t>(esz) << 36) ^ (static_cast<uint64_t>(g.L) << 8) ^ g.first_tx ^
-				(static_cast<uint64_t>(g.ntx) << 20) ^ (static_cast<uint64_t>(g.ntx_pad) << 28);
-		if (!tiled) {
-			// gather kernel below
-		} else if (scratch.bounds_version != g.grid_version || scratch.params_key != key ||
-				scratch.params_offset != g.offset) {
-			float2* b = static_cast<float2*>(scratch.bounds.reserve(static_cast<size_t>(ntiles) * g.num_elements * sizeof(float2)));
-			hipLaunchKernelGGL(tile_bounds_kernel, dim3(ntiles), dim3(256), 0, s, d_delay, g.npix, g.num_elements, tg, b);
-			HIP_CHECK(hipGetLastError());
-			const uint64_t np = static_cast<uint64_t>(ntiles) * g.nrx * g.ntx_pad;
-			WinParam* prm = static_cast<WinParam*>(scratch.params.reserve(np * sizeof(WinParam)));
-			uint32_t* d_max = static_cast<uint32_t*>(scratch.partial.reserve(256));
-			HIP_CHECK(hipMemsetAsync(d_max, 0, sizeof(uint32_t), s));
-			hipLaunchKernelGGL(window_params_kernel, dim3(static_cast<uint32_t>((np + 255) / 256)), dim3(256), 0, s, b, ntiles,
-					g.num_elements, g.first_tx, g.ntx, g.ntx_pad, g.nrx, g.L, g.offset, slot, prm, d_max);
-			HIP_CHECK(hipGetLastError());
-			uint32_t max_slots = 0;
-			HIP_CHECK(hipMemcpyAsync(&max_slots, d_max, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-			HIP_CHECK(hipStreamSynchronize(s));
-			scratch.wcap = std::max(1u, max_slots) * slot;
-			scratch.max_slots = max_slots;
-			scratch.bounds_version = g.grid_version;
-			scratch.params_key = key;
-			scratch.params_offset = g.offset;
-		}
-		// Pairs per chunk (tc, 4 or 8) and chunk buffers in the LDS ring
-		// (nbuf): the most pairs in flight, (nbuf - 1) * tc, that fit the LDS
-		// budget and the 6-bit vmcnt of the per-chunk waits.  p1 overrides:
-		// byte 0 tc, byte 1 nbuf, byte 2 LDS budget in KiB (default 76: two
-		// workgroups share a CU, which measures faster than one workgroup
-		// with a deeper ring in 150 KiB).
-		const uint32_t want_kb = (static_cast<uint32_t>(t.p1) >> 16) & 0xFF;
-		const size_t budget = static_cast<size_t>(want_kb ? std::min(want_kb, 158u) : 76u) * 1024;
-		const uint32_t S = scratch.wcap / slot;
-		auto fit = [&](uint32_t tc, uint32_t want) -> uint32_t {
-			const size_t bb = static_cast<size_t>(tc) * scratch.wcap * esz + kTileThreads * 4;
-			uint32_t nb = static_cast<uint32_t>(std::min<size_t>(16, budget / bb));
-			if (want) nb = std::min(nb, want);
-			const uint32_t per = tc * S / (kTileThreads / 64) + 1;   // copies per wave per chunk, at most
-			while (nb > 2 && (nb - 2) * per > 63) --nb;
-			return (nb >= 2 && 2 * per <= 63) ? nb : 0u;
-		};
-		const uint32_t want_tc = static_cast<uint32_t>(t.p1) & 0xFF, want_nb = (static_cast<uint32_t>(t.p1) >> 8) & 0xFF;
-		uint32_t tc = 0, nbuf = 0;
-		for (uint32_t c : {8u, 4u}) {
-			if (want_tc && c != want_tc) continue;
-			const uint32_t nb = fit(c, want_nb);
-			if (nb && (nb - 1) * c > (nbuf ? (nbuf - 1) * tc : 0u)) { tc = c; nbuf = nb; }
-		}
-		if (scratch.max_slots > 511 || nbuf == 0) tiled = false;
-		if (tiled) {
-			TileArgs A;
-			A.d = a;
-			A.tg = tg;
-			A.params = scratch.params.as<const WinParam>();
-			A.wcap = scratch.wcap;
-			A.ntx_pad = g.ntx_pad;
-			A.nbuf = nbuf;
-			switch (g.storage) {
-			case MISTA_STORAGE_F32: launch_tile_storage<MISTA_STORAGE_F32>(g.cf_kind, A, ntiles, tc, s); break;
-			case MISTA_STORAGE_F16: launch_tile_storage<MISTA_STORAGE_F16>(g.cf_kind, A, ntiles, tc, s); break;
-			case kStorageReal: launch_tile_storage<kStorageReal>(g.cf_kind, A, ntiles, tc, s); break;
-			default: launch_tile_storage<MISTA_STORAGE_BF16>(g.cf_kind, A, ntiles, tc, s); break;
+	if (can_tile) {
+		if (t.variant == 2 || t.p0 || t.p1) {
+			if (launch_tiled(g, t.p0, t.p1, scratch, a, d_delay, s)) return;
+		} else {
+			// Autotuner: on the first frame of a geometry, time every
+			// candidate tile shape and LDS budget (each writes the same
+			// images) and keep the fastest for the frames that follow.  The
+			// first run of a shape builds its window tables and is not timed.
+			const uint64_t key = g.grid_version ^ (static_cast<uint64_t>(g.L) << 20) ^ (static_cast<uint64_t>(g.ntx) << 40) ^
+					(static_cast<uint64_t>(g.nrx) << 50) ^ (static_cast<uint64_t>(g.storage) << 60) ^
+					(static_cast<uint64_t>(g.cf_kind) << 62) ^ static_cast<uint64_t>(g.offset * 1024.0f);
+			if (scratch.tuned_key != key) {
+				const std::vector<int> shapes = g.ny > 1 ? std::vector<int>{4} : std::vector<int>{1, 2, 3};
+				hipEvent_t e0 = nullptr, e1 = nullptr;
+				HIP_CHECK(hipEventCreate(&e0));
+				HIP_CHECK(hipEventCreate(&e1));
+				float best = 0.0f;
+				int bp0 = -1, bp1 = 0;
+				for (int p0 : shapes) {
+					if (!launch_tiled(g, p0, 52 << 16, scratch, a, d_delay, s)) continue;
+					for (int kb : {52, 76}) {
+						HIP_CHECK(hipEventRecord(e0, s));
+						const bool ok = launch_tiled(g, p0, kb << 16, scratch, a, d_delay, s);
+						HIP_CHECK(hipEventRecord(e1, s));
+						HIP_CHECK(hipEventSynchronize(e1));
+						float ms = 0.0f;
+						HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+						if (ok && (bp0 < 0 || ms < best)) {
+							best = ms;
+							bp0 = p0;
+							bp1 = kb << 16;
+						}
+					}
+				}
+				(void)hipEventDestroy(e0);
+				(void)hipEventDestroy(e1);
+				scratch.tuned_key = key;
+				scratch.tuned_p0 = bp0;
+				scratch.tuned_p1 = bp1;
+				// the winner runs once more so that its tables stay resident
+				if (bp0 >= 0 && launch_tiled(g, bp0, bp1, scratch, a, d_delay, s)) return;
+			} else if (scratch.tuned_p0 >= 0 &&
+					launch_tiled(g, scratch.tuned_p0, scratch.tuned_p1, scratch, a, d_delay, s)) {
+				return;
 			}
-			HIP_CHECK(hipGetLastError());
-			return;
 		}
 	}
 	switch (g.storage) {

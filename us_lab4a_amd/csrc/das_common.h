@@ -222,7 +222,7 @@ __device__ __forceinline__ void wait_vmcnt_at_most(uint32_t n)
 // the number of copies issued after chunk i is known and the wave waits for
 // chunk i alone with s_waitcnt vmcnt.
 template<int STORAGE, int CF, int NTXMAX, int TC>
-__global__ void __launch_bounds__(kTileThreads)
+__global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3, 3)))
 das_tile_kernel(const TileArgs A)
 {
 	using E = typename Entry<STORAGE>::T;

@@ -158,6 +158,13 @@ class HipSTAProcessor:
     def set_kernel(self, variant, p0=0, p1=0):
         check(lib().mista_set_kernel(self._h, variant, p0, p1), self._h)
 
+    def tuned_kernel(self):
+        """(tile shape, chunking) the automatic variant chose; (-1, 0) when
+        nothing was tuned or the gather kernel runs."""
+        a, b = ctypes.c_int32(0), ctypes.c_int32(0)
+        check(lib().mista_tuned_kernel(self._h, ctypes.byref(a), ctypes.byref(b)), self._h)
+        return a.value, b.value
+
     # -- reference interface ----------------------------------------------
     def prepare(self, base_element):
         """ArrayProcessor::prepare (VectorialSTAProcessor.h:151-156)."""
