@@ -330,3 +330,37 @@ def test_fused_prepare_shapes(shape):
     err = np.abs(a_gpu - a64).max() / m
     say("fused prepare error / peak", err)
     assert err <= 2e-6
+
+
+def _bf16_round(x):
+    """float32 -> bfloat16 (round to nearest even) -> float32, in numpy."""
+    b = x.astype(np.float32).view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) >> 16
+    return (b.astype(np.uint32) << 16).view(np.float32)
+
+
+@pytest.mark.parametrize("shape", [(16, 301, 1), (4, 4096, 8)], ids=["U1_odd", "U8_4096"])
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_fused_reduced_storage_is_scaled_fp32(shape, storage):
+    """The fused kernel writes FP16/BF16 directly: each channel must hold
+    round(z / s) with s = max(|re|, |im|) of the channel's FP32 analytic
+    signal, exactly as converting the FP32 tensor afterwards would."""
+    N, ns, up = shape
+    wl = workloads.tiny(num_elements=N, num_samples=ns, upsampling=up)
+    echoes = wl.echoes_cpu()
+    say("fused reduced storage", storage, shape)
+    out = {}
+    for st in ("f32", storage):
+        p = make_proc(wl, storage=st)
+        p.set_prep(2)
+        p.process_echoes(echoes)
+        out[st] = p.download_analytic()
+        p.close()
+    t = out["f32"]
+    s = np.abs(t).max(axis=(2, 3), keepdims=True).astype(np.float32)
+    s = np.where(s > 0, s, np.float32(1.0)).astype(np.float32)
+    inv = (np.float32(1.0) / s).astype(np.float32)
+    q = (t * inv).astype(np.float32)
+    r = q.astype(np.float16).astype(np.float32) if storage == "f16" else _bf16_round(q)
+    expect = (r * s).astype(np.float32)
+    assert np.array_equal(out[storage], expect)

@@ -232,6 +232,23 @@ __global__ void __launch_bounds__(256) hilbert_kernel(const float2* __restrict__
 
 // ---------------------------------------------------------------- fused LDS path
 
+__device__ inline uint32_t pack_bf16x2(float re, float im)
+{
+	// round to nearest even
+	auto rne = [](float f) -> uint32_t {
+		uint32_t b = __float_as_uint(f);
+		b += 0x7FFFu + ((b >> 16) & 1u);
+		return b >> 16;
+	};
+	return rne(re) | (rne(im) << 16);
+}
+
+__device__ inline uint32_t pack_f16x2(float re, float im)
+{
+	const __half2 h = __floats2half2_rn(re, im);
+	return *reinterpret_cast<const uint32_t*>(&h);
+}
+
 // One workgroup per channel, everything between the raw echo and the
 // analytic signal on chip: FIR x U + DC removal into LDS, then the
 // analytic-signal FFTs of size N = Nf (HilbertEnvelope.h:142-251) as three
@@ -248,6 +265,7 @@ __global__ void __launch_bounds__(256) hilbert_kernel(const float2* __restrict__
 // Bo waits in a global scratch row while Be is transformed.
 
 constexpr int kFftThreads = 1024;
+constexpr uint32_t kFirJ = 27;   // taps per phase of the Kaiser upsampling design (every U)
 constexpr uint32_t kFftMaxM = 18432;
 constexpr int kFftMaxPasses = 16;
 
@@ -265,12 +283,14 @@ struct AnalyticArgs {
 	const float2* wpost;   // exp(-2 pi i k / N), k in [0, M/2]
 	float2* scratch;       // [blocks][M]
 	float2* out;           // [blocks][L] analytic signal, or
-	float* out_mag;        // [blocks][L] its magnitude (kOutMagnitude)
+	float* out_mag;        // [blocks][L] its magnitude (kOutMagnitude), or
+	uint32_t* out16;       // [blocks][L] FP16/BF16 pairs over a per-channel scale
+	float* scale;          // [blocks] that scale (out16)
 	uint32_t ns, up, J, off, L, N, dz;
-	uint32_t flags;        // kRemoveDc | kOutMagnitude
+	uint32_t flags;        // kRemoveDc | kOutMagnitude | kOutBf16
 	FftPlanArgs plan;
 };
-constexpr uint32_t kRemoveDc = 1, kOutMagnitude = 2;
+constexpr uint32_t kRemoveDc = 1, kOutMagnitude = 2, kOutBf16 = 4;
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
@@ -394,6 +414,7 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 {
 	extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
 	__shared__ double red[kFftThreads / 64];
+	__shared__ float redm[kFftThreads / 64];
 	const uint32_t tid = threadIdx.x, ch = blockIdx.x;
 	const uint32_t M = A.plan.M, N = A.N, L = A.L, up = A.up, J = A.J;
 	float2* D = reinterpret_cast<float2*>(lds);
@@ -414,7 +435,32 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 		__syncthreads();
 	}
 	double part = 0.0;
-	for (uint32_t i = tid; i < L; i += kFftThreads) {
+	if (up > 1 && J == kFirJ) {
+		// Lane per input position q: the window x[q-26..q] sits in registers
+		// and serves all U phases (outputs m = q*U + p, i = m - off); the
+		// taps, the same for every lane, come through the scalar cache.  Each
+		// output is the same fma chain over j = 0..J-1 as below.
+		const float* __restrict__ hg = A.hpoly;
+		const uint32_t q_lo = A.off / up, q_hi = (L - 1 + A.off) / up;
+		for (uint32_t q = q_lo + tid; q <= q_hi; q += kFftThreads) {
+			float xw[kFirJ];
+			const float* xrow = xs + J + q;
+#pragma unroll
+			for (int j = 0; j < kFirJ; ++j) xw[j] = xrow[-j];
+			const int ibase = static_cast<int>(q * up) - static_cast<int>(A.off);
+			for (uint32_t ph = 0; ph < up; ++ph) {
+				const float* hrow = hg + ph * kFirJ;
+				float acc = 0.0f;
+#pragma unroll
+				for (int j = 0; j < kFirJ; ++j) acc = fmaf(hrow[j], xw[j], acc);
+				const int i = ibase + static_cast<int>(ph);
+				if (i >= 0 && i < static_cast<int>(L)) {
+					if (static_cast<uint32_t>(i) >= A.dz) part += static_cast<double>(acc);
+					Df[i] = acc;
+				}
+			}
+		}
+	} else for (uint32_t i = tid; i < L; i += kFftThreads) {
 		float acc;
 		if (up > 1) {
 			const uint32_t m = i + A.off;
@@ -494,6 +540,58 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 		else dst[n] = make_float2(re, im);
 	};
 	fft_forward(D, A.plan, A.tw);
+	if (A.out16) {
+		// FP16/BF16 storage over the channel scale s = max(|re|, |im|) (1 if
+		// zero), stored value z/s.  The even samples wait, in FP32, in the
+		// words of the channel's own output row that their pair will occupy
+		// (word 2m, 2m+1 <- z[2m]), while the odd transform runs; each thread
+		// later reads back only what it wrote.  An odd L's last even sample
+		// stays in a register.
+		float* park = reinterpret_cast<float*>(A.out16 + static_cast<uint64_t>(ch) * L);
+		float mx = 0.0f;
+		float2 last = make_float2(0.0f, 0.0f);
+		for (uint32_t m = tid; 2 * m < L; m += kFftThreads) {
+			const float2 v = D[m];
+			const float re = v.x * coef, im = -v.y * coef;
+			mx = fmaxf(mx, fmaxf(fabsf(re), fabsf(im)));
+			if (2 * m + 1 < L) {
+				park[2 * m] = re;
+				park[2 * m + 1] = im;
+			} else {
+				last = make_float2(re, im);
+			}
+		}
+		__syncthreads();
+		for (uint32_t m = tid; m < M; m += kFftThreads) D[m] = bo[m];
+		__syncthreads();
+		fft_forward(D, A.plan, A.tw);
+		for (uint32_t m = tid; 2 * m + 1 < L; m += kFftThreads) {
+			const float2 v = D[m];
+			mx = fmaxf(mx, fmaxf(fabsf(v.x * coef), fabsf(-v.y * coef)));
+		}
+		for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+		if ((tid & 63) == 0) redm[tid >> 6] = mx;
+		__syncthreads();
+		float gm = redm[0];
+		for (int w = 1; w < kFftThreads / 64; ++w) gm = fmaxf(gm, redm[w]);
+		const float sc = (gm > 0.0f) ? gm : 1.0f;
+		const float inv = 1.0f / sc;
+		if (tid == 0) A.scale[ch] = sc;
+		const bool bf = (A.flags & kOutBf16) != 0;
+		uint32_t* row = A.out16 + static_cast<uint64_t>(ch) * L;
+		for (uint32_t m = tid; 2 * m < L; m += kFftThreads) {
+			if (2 * m + 1 < L) {
+				const float er = park[2 * m], ei = park[2 * m + 1];
+				const float2 v = D[m];
+				const float orr = v.x * coef, oi = -v.y * coef;
+				row[2 * m] = bf ? pack_bf16x2(er * inv, ei * inv) : pack_f16x2(er * inv, ei * inv);
+				row[2 * m + 1] = bf ? pack_bf16x2(orr * inv, oi * inv) : pack_f16x2(orr * inv, oi * inv);
+			} else {
+				row[2 * m] = bf ? pack_bf16x2(last.x * inv, last.y * inv) : pack_f16x2(last.x * inv, last.y * inv);
+			}
+		}
+		return;
+	}
 	for (uint32_t m = tid; 2 * m < L; m += kFftThreads) put(2 * m, D[m]);
 	__syncthreads();
 	for (uint32_t m = tid; m < M; m += kFftThreads) D[m] = bo[m];
@@ -504,16 +602,6 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 
 // ---------------------------------------------------------------- storage
 
-__device__ inline uint32_t pack_bf16x2(float re, float im)
-{
-	// round to nearest even
-	auto rne = [](float f) -> uint32_t {
-		uint32_t b = __float_as_uint(f);
-		b += 0x7FFFu + ((b >> 16) & 1u);
-		return b >> 16;
-	};
-	return rne(re) | (rne(im) << 16);
-}
 
 // One workgroup per channel.  F32: copy.  F16/BF16: per-channel scale
 // s = max(|re|, |im|) over the channel (1 if zero), stored value z/s.
@@ -772,8 +860,7 @@ static void prepare_fused(PrepState& st, const FrameGeom& g, const float* d_raw,
 	if (P.fused_nf != g.nf) build_fused_plan(P, g.nf, s);
 	const uint32_t M = g.nf / 2;
 	float2* scratch = static_cast<float2*>(P.spec.reserve(static_cast<size_t>(chunk) * M * sizeof(float2)));
-	float2* tmp = (g.storage == MISTA_STORAGE_F32) ? nullptr
-			: static_cast<float2*>(P.z.reserve(static_cast<size_t>(chunk) * g.L * sizeof(float2)));
+	const bool scaled = g.storage != MISTA_STORAGE_F32;
 	HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(analytic_lds_kernel),
 			hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
 	AnalyticArgs A{};
@@ -788,24 +875,23 @@ static void prepare_fused(PrepState& st, const FrameGeom& g, const float* d_raw,
 	A.L = g.L;
 	A.N = g.nf;
 	A.dz = g.dz_up;
-	A.flags = kRemoveDc;
+	A.flags = kRemoveDc | (g.storage == MISTA_STORAGE_BF16 ? kOutBf16 : 0u);
 	A.out_mag = nullptr;
 	A.plan = P.fplan;
 	for (uint32_t c0 = 0; c0 < nch; c0 += chunk) {
 		const uint32_t n = std::min(chunk, nch - c0);
 		A.raw = d_raw + static_cast<uint64_t>(c0) * g.ns;
-		A.out = tmp ? tmp : static_cast<float2*>(d_tensor) + static_cast<uint64_t>(c0) * g.L;
+		if (scaled) {
+			A.out = nullptr;
+			A.out16 = static_cast<uint32_t*>(d_tensor) + static_cast<uint64_t>(c0) * g.L;
+			A.scale = d_scale + c0;
+		} else {
+			A.out = static_cast<float2*>(d_tensor) + static_cast<uint64_t>(c0) * g.L;
+			A.out16 = nullptr;
+			A.scale = nullptr;
+		}
 		hipLaunchKernelGGL(analytic_lds_kernel, dim3(n), dim3(kFftThreads), lds, s, A);
 		HIP_CHECK(hipGetLastError());
-		if (tmp) {
-			if (g.storage == MISTA_STORAGE_F16)
-				hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_F16>, dim3(n), dim3(256), 0, s, tmp, g.L, g.L, d_tensor, d_scale,
-						static_cast<uint64_t>(c0));
-			else
-				hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_BF16>, dim3(n), dim3(256), 0, s, tmp, g.L, g.L, d_tensor, d_scale,
-						static_cast<uint64_t>(c0));
-			HIP_CHECK(hipGetLastError());
-		}
 	}
 }
 
