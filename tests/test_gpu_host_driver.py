@@ -98,3 +98,51 @@ def test_cli_hdf5_acquisition_and_images(tmp_path):
     cfg.cf_kind = oracle.CF_SCF
     v_o, _, _, _ = oracle.process(cfg, echoes, imaging.grid_xz(g), wl.rx_apodization())
     assert float(np.abs(v_raw - v_o).max()) / float(np.abs(v_o).max()) <= 1e-4
+
+
+def test_cli_project_config_matches_oracle(tmp_path):
+    """The driver configured by the reference's key=value project files
+    (STA, imaging, coherence factor, rectangular grid) images the raw echo
+    file like the oracle on the grid those files describe."""
+    if not os.path.exists(CLI):
+        pytest.fail("host driver not built (python -m us_lab4a_amd.build)")
+    wl = workloads.sta2d_32_small()
+    wl.tx_apod = None
+    wl.cf = "phase_coherence_factor"
+    echoes = wl.echoes_cpu()
+    path = tmp_path / "echoes.f32"
+    echoes.astype(np.float32).tofile(path)
+    n = wl.num_elements
+    (tmp_path / "sta.txt").write_text(
+        f"num_elements = {n}\nnum_elements_mux = {n}\nfirst_tx_elem = 0\nlast_tx_elem = {n - 1}\nnum_pulses = 1\n"
+        f"pitch = {wl.pitch!r}\ncenter_frequency = {wl.fc!r}\nmax_frequency = 10.0e6\nacquisition_time = 1.0e-4\n"
+        f"min_gain = 0.0\nmax_gain = 0.0\npropagation_speed_1 = {wl.c!r}\nacquisition_delay = 0.0\n"
+        f"sampling_frequency = {wl.fs!r}\ndead_zone_m = 0.0\nvalue_scale = 0.0\n")
+    (tmp_path / "img.txt").write_text(f"peak_offset = 0.0\nupsampling_factor = {wl.upsampling}\nrx_apodization = hanning\n")
+    (tmp_path / "cf.txt").write_text("analytic_signal_coherence_factor_method = phase_coherence_factor\n"
+                                     "phase_coherence_factor_gamma = 1.0\n")
+    (tmp_path / "grid.txt").write_text(
+        "image_grid_type = rectangular\nrectangular_min_x = -5.0e-3\nrectangular_max_x = 5.0e-3\n"
+        "rectangular_min_z = 10.0e-3\nrectangular_max_z = 30.0e-3\nrectangular_step_div = 0.5\n"
+        "rectangular_origin_x = 0.0\nrectangular_origin_z = 0.0\n")
+    cfg = ["--sta-config", str(tmp_path / "sta.txt"), "--imag-config", str(tmp_path / "img.txt"),
+           "--cf-config", str(tmp_path / "cf.txt"), "--grid-config", str(tmp_path / "grid.txt")]
+    dump = tmp_path / "dump"
+    dump.mkdir()
+    r = subprocess.run([CLI] + cfg + ["--dump-config", str(dump)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    n1, n2 = map(int, dict(l.split(" = ", 1) for l in open(dump / "config.txt").read().splitlines())["grid"].split())
+    out = tmp_path / "img"
+    print("[gpu-test] cli with project files", n1, n2, flush=True)
+    r = subprocess.run([CLI] + cfg + ["--echoes", str(path), "--samples", str(wl.num_samples), "--out", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    sys.stdout.write(r.stdout)
+    assert r.returncode == 0, r.stderr
+    v = np.fromfile(str(out) + "_value.f32", np.float32).reshape(n1, n2)
+    f = np.fromfile(str(out) + "_factor.f32", np.float32).reshape(n1, n2)
+    xz = np.stack([np.fromfile(dump / "grid_x.f64").astype(np.float32),
+                   np.fromfile(dump / "grid_z.f64").astype(np.float32)], axis=-1).reshape(n1, n2, 2)
+    v_o, f_o, _, _ = oracle.process(wl.oracle_cfg(), echoes, xz, wl.rx_apodization())
+    peak = float(np.abs(v_o).max())
+    assert float(np.abs(v - v_o).max()) / peak <= 1e-4
+    assert float(np.abs(v * f - v_o * f_o).max()) / peak <= 1e-4
