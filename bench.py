@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--workload", default="sta2d_128_pcf")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--kernel", type=int, default=0, help="DAS variant (0 auto, 1 gather, 2 tiled)")
+    ap.add_argument("--tile", type=int, nargs=2, metavar=("SHAPE", "CHUNKING"), default=None,
+                    help="fixed tiled-DAS parameters instead of the autotuner (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--mode", default="frame", choices=["frame", "batch", "serve"],
@@ -264,7 +266,9 @@ def main():
     proc = HipSTAProcessor(wl.sta_config(), None, wl.upsampling, CoherenceFactor(wl.cf, wl.cf_param),
                            wl.peak_offset, wl.rx_apodization(), tx_apod=wl.tx_apodization(), storage=args.storage)
     proc.set_grid(wl.grid())
-    if args.kernel:
+    if args.tile:
+        proc.set_kernel(2, *args.tile)
+    elif args.kernel:
         proc.set_kernel(args.kernel)
     d_value = torch.empty(npix, dtype=torch.float32, device=dev)
     d_factor = torch.empty(npix, dtype=torch.float32, device=dev)
@@ -316,7 +320,7 @@ def main():
         "config": {"workload": wl.name, "elements": wl.num_elements, "tx": wl.num_tx, "samples": wl.num_samples,
                    "upsampling": wl.upsampling, "grid": [wl.n_x, wl.n_z], "coherence": wl.cf,
                    "storage": args.storage, "pixel_pairs_per_frame": wl.pixel_pairs,
-                   "das_tile": tile_name(proc.tuned_kernel())},
+                   "das_tile": tile_name(tuple(args.tile) if args.tile else proc.tuned_kernel())},
         "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                              "p99": round(float(np.percentile(lat, 99)), 4)},
         "stage_ms": {"prep": round(prep_avg, 4), "das": round(das_avg, 4)},
@@ -340,9 +344,10 @@ def tile_name(tuned):
     """Human-readable tile choice of the DAS autotuner."""
     p0, p1 = tuned
     if p0 < 0:
-        return "gather"
+        return "gather kernel"
     shape = {2: "8x32", 3: "32x8", 4: "8x8x4"}.get(p0, "16x16")
-    return f"{shape} tile, {(p1 >> 16) & 0xFF} KiB LDS ring"
+    kb = (p1 >> 16) & 0xFF
+    return f"{shape} tile, {kb or 'default'} KiB LDS ring"
 
 
 if __name__ == "__main__":

@@ -43,9 +43,9 @@ def _pmc(d, counter):
     if not files:
         sys.exit(f"no counter_collection.csv under {d}")
     vals = {}
-    for r in csv.DictReader(open(files[0])):
-        if r["Counter_Name"] == counter:
-            vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    rows = [r for r in csv.DictReader(open(files[0])) if r["Counter_Name"] == counter]
+    for r in sorted(rows, key=lambda r: int(r["Dispatch_Id"])):
+        vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
     return vals
 
 
