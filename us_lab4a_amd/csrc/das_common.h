@@ -269,17 +269,17 @@ das_tile_kernel(const TileArgs A)
 		dtx[t] = (static_cast<uint32_t>(t) < ntx) ? a.offset + d : -1.0e30f;
 	}
 
-	// Issue the copies of chunk i into buffer b.  Slot sl of pair j is copy
-	// number j * S + sl of the chunk and goes to wave (j * S + sl) % NW.
+	// Issue the copies of chunk i into buffer b, given its window parameters
+	// pp.  Slot sl of pair j is copy number j * S + sl of the chunk and goes
+	// to wave (j * S + sl) % NW.
 	const uint32_t nchunks = nrx * CPR;
-	auto issue = [&](uint32_t i, uint32_t b) {
+	auto issue = [&](uint32_t i, uint32_t b, const WinParam* pp) {
 		const uint32_t r = i / CPR, c = i % CPR;
-		const cu2_t pr = prm + r * NTXMAX + c * TC;
 		const uint32_t bbase = lds0 + b * buf_bytes;
 #pragma unroll
 		for (uint32_t j = 0; j < TC; ++j) {
-			const uint32_t slots = (pr[j].word >> 21) & 511u;
-			const unsigned char* src = tensor + static_cast<uint64_t>(pr[j].start) * ESZ + lane * 16;
+			const uint32_t slots = (pp[j].word >> 21) & 511u;
+			const unsigned char* src = tensor + static_cast<uint64_t>(pp[j].start) * ESZ + lane * 16;
 			const uint32_t dst = bbase + j * wcap * ESZ;
 			for (uint32_t sl = (wave - j * S) & (NW - 1); sl < S; sl += NW) {
 				const uint32_t from = sl < slots ? sl : 0u;
@@ -295,8 +295,25 @@ das_tile_kernel(const TileArgs A)
 		reinterpret_cast<uint4*>(smem)[i] = make_uint4(0u, 0u, 0u, 0u);
 	__syncthreads();
 
+	// Window parameters of the next chunk to issue, loaded one chunk ahead:
+	// their scalar loads miss the caches (each tile streams its own table),
+	// and loading them at the issue itself would add that latency to the
+	// copies' on every chunk.
+	WinParam nx[TC];
+	auto load_params = [&](uint32_t k) {
+		const cu2_t pr = prm + (k / CPR) * NTXMAX + (k % CPR) * TC;
+#pragma unroll
+		for (uint32_t j = 0; j < TC; ++j) {
+			nx[j].start = pr[j].start;
+			nx[j].word = pr[j].word;
+		}
+	};
 	const uint32_t pre = min(NB - 1, nchunks);
-	for (uint32_t k = 0; k < pre; ++k) issue(k, k);
+	for (uint32_t k = 0; k < pre; ++k) {
+		load_params(k);
+		issue(k, k, nx);
+	}
+	if (pre < nchunks) load_params(pre);
 
 	PixelStats<CF> st;
 	uint32_t i = 0, b = 0;
@@ -310,7 +327,7 @@ das_tile_kernel(const TileArgs A)
 			const uint32_t after = (hi > i) ? (hi - i) * QW + (hi / CPR - i / CPR) : 0u;
 			wait_vmcnt_at_most(after);         // this wave's copies of chunk i landed
 			__syncthreads();                   // everyone's did; buffer of chunk i-1 is free
-			if (i + NB - 1 < nchunks) issue(i + NB - 1, (b + NB - 1) % NB);
+			if (i + NB - 1 < nchunks) issue(i + NB - 1, (b + NB - 1) % NB, nx);
 			const uint32_t bbase = b * buf_bytes;
 			if (c == 0) drx = *reinterpret_cast<const float*>(smem + bbase + win_bytes + tid * 4);
 			// Register fences: this chunk's positions are computed after the
@@ -345,6 +362,12 @@ das_tile_kernel(const TileArgs A)
 				ai = fmaf(fmaf(k, v1.y - v0.y, v0.y), w, ai);
 			}
 			asm volatile("" : "+v"(ar), "+v"(ai));
+			// after this chunk's LDS reads (a pending scalar load would make
+			// every LDS wait a full drain); the chunk index passes through a
+			// fence so the loads stay here
+			uint32_t nk = i + NB;
+			asm volatile("" : "+s"(nk));
+			if (nk < nchunks) load_params(nk);
 			b = (b + 1 == NB) ? 0u : b + 1;
 		}
 		st.add_rx(ar, ai, r == 0);
