@@ -345,7 +345,7 @@ def tile_name(tuned):
     p0, p1 = tuned
     if p0 < 0:
         return "gather kernel"
-    shape = {2: "8x32", 3: "32x8", 4: "8x8x4"}.get(p0, "16x16")
+    shape = {2: "8x32", 3: "32x8", 4: "8x8x4", 5: "16x32", 6: "32x16", 7: "8x8x8"}.get(p0, "16x16")
     kb = (p1 >> 16) & 0xFF
     return f"{shape} tile, {kb or 'default'} KiB LDS ring"
 

@@ -96,16 +96,17 @@ def test_small_volume_matches_oracle(small, cf):
     assert bad.mean() <= 0.01
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_volume_tiles_match_gather(small, storage):
-    """The 8x8x8 voxel-tile kernel and the one-voxel-per-lane gather kernel
-    form bit-identical volumes (same per-pair arithmetic, same order)."""
+@pytest.mark.parametrize("storage,shape", [("f32", 4), ("f16", 4), ("f32", 7)])
+def test_volume_tiles_match_gather(small, storage, shape):
+    """The voxel-tile kernels (8x8x4 on 256 lanes, 8x8x8 on 512) and the
+    one-voxel-per-lane gather kernel form bit-identical volumes (same
+    per-pair arithmetic, same order)."""
     wl, echoes = small
     wl = workloads.sta3d_small(n=(16, 16, 40))
     echoes = wl.echoes_cpu()
-    say("3-D tiles vs gather", storage)
+    say("3-D tiles vs gather", storage, shape)
     p = make_proc(wl, storage=storage)
-    p.set_kernel(2, 4)
+    p.set_kernel(2, shape)
     v_t, f_t = p.process_echoes(echoes)
     p.set_kernel(1)
     v_g, f_g = p.process_echoes(echoes)

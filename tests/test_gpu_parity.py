@@ -241,9 +241,12 @@ def test_graded_shape_columns_match_oracle():
     p.close()
 
 
-@pytest.mark.parametrize("tz,storage", [(0, "f32"), (2, "f32"), (3, "f32"), (0, "f16"), (0, "bf16")])
-def test_tiled_kernel_equals_gather_kernel(small, tz, storage):
-    """The LDS-tiled kernel uses the gather kernel's arithmetic and, with a
+@pytest.mark.parametrize("tz,storage,chunking", [(0, "f32", 4), (2, "f32", 4), (3, "f32", 4), (0, "f16", 4),
+                                                 (0, "bf16", 4), (5, "f32", 8 | (150 << 16)),
+                                                 (6, "f16", 8 | (150 << 16))])
+def test_tiled_kernel_equals_gather_kernel(small, tz, storage, chunking):
+    """The LDS-tiled kernel (256-lane tiles 16x16, 8x32, 32x8 and 512-lane
+    tiles 16x32, 32x16) uses the gather kernel's arithmetic and, with a
     coherence factor, its summation order: results are bit-identical."""
     import torch
     wl, echoes = small
@@ -254,7 +257,7 @@ def test_tiled_kernel_equals_gather_kernel(small, tz, storage):
     n = wl.n_x * wl.n_z
     out = {}
     for variant, p0 in ((1, 0), (2, tz)):
-        p.set_kernel(variant, p0, 4)
+        p.set_kernel(variant, p0, chunking)
         dv = torch.empty(n, dtype=torch.float32, device="cuda")
         df = torch.empty(n, dtype=torch.float32, device="cuda")
         p.das_device(dv, df)
@@ -286,7 +289,7 @@ def test_autotuned_kernel_equals_gather_kernel(small):
         if variant == 0:
             tuned = p.tuned_kernel()
             say("autotuner chose", tuned)
-            assert tuned[0] in (1, 2, 3) and (tuned[1] >> 16) in (52, 76)
+            assert tuned[0] in (1, 2, 3, 5, 6) and (tuned[1] >> 16) in (52, 76, 150)
     for v, f in outs[:2]:
         assert np.array_equal(v, outs[2][0]) and np.array_equal(f, outs[2][1])
     p.close()

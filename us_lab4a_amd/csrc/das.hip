@@ -195,7 +195,9 @@ __global__ void __launch_bounds__(256) window_params_kernel(const float2* __rest
 
 // Builds (or reuses) the window tables for tile shape p0 and launches the
 // tiled kernel with the chunking p1.  p0: 0/1 = 16x16 (planes) or 8x8x4
-// (volumes), 2 = 8x32, 3 = 32x8, 4 = 8x8x4.  p1: byte 0 pairs per chunk
+// (volumes), 2 = 8x32, 3 = 32x8, 4 = 8x8x4; 512-point tiles (512-lane
+// workgroups, 8 pairs per chunk): 5 = 16x32, 6 = 32x16, 7 = 8x8x8.  p1:
+// byte 0 pairs per chunk
 // (4 or 8), byte 1 chunk buffers, byte 2 LDS budget in KiB (default: the
 // smallest of 52 (three workgroups per CU: the kernel is compiled for three
 // waves per SIMD), 76 (two) and 150 (one) in which the windows fit).
@@ -209,6 +211,10 @@ bool launch_tiled(const FrameGeom& g, int p0, int p1, DasScratch& scratch, const
 	if (p0 == 2) { tw1 = 8; tw2 = 1; tzh = 32; }
 	else if (p0 == 3) { tw1 = 32; tw2 = 1; tzh = 8; }
 	else if (p0 == 4) { tw1 = 8; tw2 = 8; tzh = 4; }
+	else if (p0 == 5) { tw1 = 16; tw2 = 1; tzh = 32; }
+	else if (p0 == 6) { tw1 = 32; tw2 = 1; tzh = 16; }
+	else if (p0 == 7) { tw1 = 8; tw2 = 8; tzh = 8; }
+	const uint32_t nt = tw1 * tw2 * tzh, nw = nt / 64;
 	const uint32_t esz = static_cast<uint32_t>(storage_bytes(g.storage));
 	const uint32_t slot = 64 * (16 / esz);
 	const uint32_t nx = g.n1 / ny;
@@ -252,15 +258,17 @@ bool launch_tiled(const FrameGeom& g, int p0, int p1, DasScratch& scratch, const
 		if (want_kb && kb != 52u) break;
 		const size_t budget = static_cast<size_t>(want_kb ? std::min(want_kb, 158u) : kb) * 1024;
 		auto fit = [&](uint32_t c) -> uint32_t {
-			const size_t bb = static_cast<size_t>(c) * scratch.wcap * esz + kTileThreads * 4;
+			const size_t bb = static_cast<size_t>(c) * scratch.wcap * esz + nt * 4;
 			uint32_t nb = static_cast<uint32_t>(std::min<size_t>(16, budget / bb));
 			if (want_nb) nb = std::min(nb, want_nb);
-			const uint32_t per = c * S / (kTileThreads / 64) + 1;   // copies per wave per chunk, at most
+			if ((c * S) % nw != 0) return 0u;   // every wave must issue the same number of copies
+			const uint32_t per = c * S / nw + 1;   // copies per wave per chunk, at most
 			while (nb > 2 && (nb - 2) * per > 63) --nb;
 			return (nb >= 2 && 2 * per <= 63) ? nb : 0u;
 		};
 		for (uint32_t c : {8u, 4u}) {
 			if (want_tc && c != want_tc) continue;
+			if (nt == 512 && c != 8) continue;
 			const uint32_t nb = fit(c);
 			if (nb && (nb - 1) * c > (nbuf ? (nbuf - 1) * tc : 0u)) { tc = c; nbuf = nb; }
 		}
@@ -330,15 +338,20 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 					(static_cast<uint64_t>(g.nrx) << 50) ^ (static_cast<uint64_t>(g.storage) << 60) ^
 					(static_cast<uint64_t>(g.cf_kind) << 62) ^ static_cast<uint64_t>(g.offset * 1024.0f);
 			if (scratch.tuned_key != key) {
-				const std::vector<int> shapes = g.ny > 1 ? std::vector<int>{4} : std::vector<int>{1, 2, 3};
+				const std::vector<int> shapes = g.ny > 1 ? std::vector<int>{4, 7} : std::vector<int>{1, 2, 3, 5, 6};
 				hipEvent_t e0 = nullptr, e1 = nullptr;
 				HIP_CHECK(hipEventCreate(&e0));
 				HIP_CHECK(hipEventCreate(&e1));
 				float best = 0.0f;
 				int bp0 = -1, bp1 = 0;
 				for (int p0 : shapes) {
-					if (!launch_tiled(g, p0, 52 << 16, scratch, a, d_delay, s)) continue;
-					for (int kb : {52, 76}) {
+					// 256-lane tiles: three or two workgroups per CU; 512-lane
+					// tiles: one
+					const int kbs[2] = {p0 >= 5 ? 76 : 52, p0 >= 5 ? 150 : 76};
+					if (!launch_tiled(g, p0, kbs[0] << 16, scratch, a, d_delay, s) &&
+							!launch_tiled(g, p0, kbs[1] << 16, scratch, a, d_delay, s))
+						continue;
+					for (int kb : kbs) {
 						HIP_CHECK(hipEventRecord(e0, s));
 						const bool ok = launch_tiled(g, p0, kb << 16, scratch, a, d_delay, s);
 						HIP_CHECK(hipEventRecord(e1, s));

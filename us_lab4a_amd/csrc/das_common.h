@@ -221,12 +221,12 @@ __device__ __forceinline__ void wait_vmcnt_at_most(uint32_t n)
 // narrower than the widest re-copy their first slot into unused space), so
 // the number of copies issued after chunk i is known and the wave waits for
 // chunk i alone with s_waitcnt vmcnt.
-template<int STORAGE, int CF, int NTXMAX, int TC>
-__global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3, 3)))
+template<int STORAGE, int CF, int NTXMAX, int TC, int NTHREADS>
+__global__ void __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(3, 3)))
 das_tile_kernel(const TileArgs A)
 {
 	using E = typename Entry<STORAGE>::T;
-	constexpr uint32_t NT = kTileThreads;
+	constexpr uint32_t NT = NTHREADS;       // one tile point per lane
 	constexpr uint32_t ESZ = sizeof(E);
 	constexpr uint32_t EPL = 16 / ESZ;      // entries per 16-byte lane load
 	constexpr uint32_t SLOT = 64 * EPL;     // entries per wave load instruction
@@ -378,22 +378,24 @@ das_tile_kernel(const TileArgs A)
 
 // LDS bytes of one chunk buffer: TC windows of wcap entries + the rx delays.
 template<int STORAGE>
-inline size_t tile_buffer_bytes(uint32_t tc, uint32_t wcap)
+inline size_t tile_buffer_bytes(uint32_t tc, uint32_t wcap, uint32_t nt)
 {
-	return static_cast<size_t>(tc) * wcap * sizeof(typename Entry<STORAGE>::T) + kTileThreads * 4;
+	return static_cast<size_t>(tc) * wcap * sizeof(typename Entry<STORAGE>::T) + nt * 4;
 }
 
 template<int STORAGE, int CF, int NTXMAX>
 void launch_tile_t(const TileArgs& A, uint32_t ntiles, uint32_t tc, hipStream_t s)
 {
-	const size_t lds = A.nbuf * tile_buffer_bytes<STORAGE>(tc, A.wcap);
-	auto go = [&](auto kern) {
+	const uint32_t nt = A.tg.tw1 * A.tg.tw2 * A.tg.tzh;   // 256 or 512 (512: tc = 8)
+	const size_t lds = A.nbuf * tile_buffer_bytes<STORAGE>(tc, A.wcap, nt);
+	auto go = [&](auto kern, uint32_t threads) {
 		HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
 				static_cast<int>(lds)));
-		hipLaunchKernelGGL(kern, dim3(ntiles), dim3(kTileThreads), lds, s, A);
+		hipLaunchKernelGGL(kern, dim3(ntiles), dim3(threads), lds, s, A);
 	};
-	if (tc == 8) go(das_tile_kernel<STORAGE, CF, NTXMAX, 8>);
-	else go(das_tile_kernel<STORAGE, CF, NTXMAX, 4>);
+	if (nt == 512) go(das_tile_kernel<STORAGE, CF, NTXMAX, 8, 512>, 512);
+	else if (tc == 8) go(das_tile_kernel<STORAGE, CF, NTXMAX, 8, 256>, 256);
+	else go(das_tile_kernel<STORAGE, CF, NTXMAX, 4, 256>, 256);
 }
 
 template<int STORAGE, int CF>
