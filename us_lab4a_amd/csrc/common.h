@@ -108,8 +108,7 @@ inline uint32_t pad_ntx(uint32_t ntx)
 {
 	if (ntx <= 32) return 32;
 	if (ntx <= 64) return 64;
-	if (ntx <= 128) return 128;
-	return (ntx + 7) / 8 * 8;
+	return (ntx + 127) / 128 * 128;   // above 128: whole blocks of 128 (the tiled DAS runs them in turn)
 }
 
 // Bytes per complex sample of a storage kind.

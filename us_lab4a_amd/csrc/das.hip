@@ -215,6 +215,7 @@ bool launch_tiled(const FrameGeom& g, int p0, int p1, DasScratch& scratch, const
 	else if (p0 == 6) { tw1 = 32; tw2 = 1; tzh = 16; }
 	else if (p0 == 7) { tw1 = 8; tw2 = 8; tzh = 8; }
 	const uint32_t nt = tw1 * tw2 * tzh, nw = nt / 64;
+	if (g.ntx_pad > 128 && nt != 256) return false;   // several transmit blocks: 256-lane tiles only
 	const uint32_t esz = static_cast<uint32_t>(storage_bytes(g.storage));
 	const uint32_t slot = 64 * (16 / esz);
 	const uint32_t nx = g.n1 / ny;
@@ -269,6 +270,7 @@ bool launch_tiled(const FrameGeom& g, int p0, int p1, DasScratch& scratch, const
 		for (uint32_t c : {8u, 4u}) {
 			if (want_tc && c != want_tc) continue;
 			if (nt == 512 && c != 8) continue;
+			if (g.ntx_pad > 128 && c != 8) continue;   // several transmit blocks: 8 pairs per chunk
 			const uint32_t nb = fit(c);
 			if (nb && (nb - 1) * c > (nbuf ? (nbuf - 1) * tc : 0u)) { tc = c; nbuf = nb; }
 		}
@@ -324,7 +326,7 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 	a.signed_value = (g.storage == kStorageReal) ? 1 : 0;
 
 	// window start entries are u32 (channel * L + ws): tensor below 2^32 entries
-	const bool can_tile = (t.variant != 1) && g.ntx_pad <= 128 && g.L < (1u << 21) - 64 &&
+	const bool can_tile = (t.variant != 1) && (g.ntx_pad <= 128 || g.ntx_pad % 128 == 0) && g.L < (1u << 21) - 64 &&
 			static_cast<uint64_t>(g.ntx) * g.nrx * g.L < (1ull << 32) - (1ull << 22);
 	if (can_tile) {
 		if (t.variant == 2 || t.p0 || t.p1) {

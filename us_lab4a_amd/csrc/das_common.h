@@ -221,7 +221,7 @@ __device__ __forceinline__ void wait_vmcnt_at_most(uint32_t n)
 // narrower than the widest re-copy their first slot into unused space), so
 // the number of copies issued after chunk i is known and the wave waits for
 // chunk i alone with s_waitcnt vmcnt.
-template<int STORAGE, int CF, int NTXMAX, int TC, int NTHREADS>
+template<int STORAGE, int CF, int NTXMAX, int TC, int NTHREADS, bool MULTI>
 __global__ void __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(3, 3)))
 das_tile_kernel(const TileArgs A)
 {
@@ -231,7 +231,7 @@ das_tile_kernel(const TileArgs A)
 	constexpr uint32_t EPL = 16 / ESZ;      // entries per 16-byte lane load
 	constexpr uint32_t SLOT = 64 * EPL;     // entries per wave load instruction
 	constexpr uint32_t NW = NT / 64;
-	constexpr uint32_t CPR = NTXMAX / TC;   // chunks per rx
+	constexpr uint32_t CPB = NTXMAX / TC;   // chunks per block of NTXMAX transmits
 	extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
 
 	const DasArgs& a = A.d;
@@ -248,33 +248,40 @@ das_tile_kernel(const TileArgs A)
 	const uint64_t npix = a.npix;
 	const uint64_t p = static_cast<uint64_t>(col) * A.tg.n2 + row;
 	const uint32_t L = a.L, ntx = a.ntx, nrx = a.nrx;
+	// transmits run in blocks of NTXMAX (MULTI: several, at NTXMAX = 128)
+	const uint32_t nblk = MULTI ? A.ntx_pad / NTXMAX : 1u, CPR = nblk * CPB;   // chunks per rx
 	const float lim = static_cast<float>(L - 1u);
 	const uint32_t wcap = A.wcap, NB = A.nbuf;
 	const uint32_t S = wcap / SLOT;                   // slots of the widest window
 	const uint32_t QW = TC * S / NW;                  // window copies per wave per chunk
 	const uint32_t win_bytes = TC * wcap * ESZ;
 	const uint32_t buf_bytes = win_bytes + NT * 4;    // windows + rx delays
-	const cu2_t prm = (cu2_t)(A.params + static_cast<uint64_t>(tile) * nrx * NTXMAX);
+	const cu2_t prm = (cu2_t)(A.params + static_cast<uint64_t>(tile) * nrx * A.ntx_pad);
 	const unsigned char* tensor = static_cast<const unsigned char*>(a.tensor);
 	const uint32_t lds0 = static_cast<uint32_t>(reinterpret_cast<size_t>((__attribute__((address_space(3))) void*)smem));
 
-	// tx delays of this lane's point, offset folded in: (offset + d_tx).
-	// Transmits are padded to NTXMAX: padded ones have weight 0, an empty
-	// window and a position far below 0, so they take the excluded path.
+	// tx delays of this lane's point for transmit block blk, offset folded
+	// in: (offset + d_tx).  Transmits are padded to whole blocks: padded ones
+	// have weight 0, an empty window and a position far below 0, so they
+	// take the excluded path.
 	float dtx[NTXMAX];
+	auto load_dtx = [&](uint32_t blk) {
 #pragma unroll
-	for (int t = 0; t < NTXMAX; ++t) {
-		const uint32_t tt = min(static_cast<uint32_t>(t), ntx - 1);   // unconditional load, no branch
-		const float d = a.delay[static_cast<uint64_t>(a.first_tx + tt) * npix + p];
-		dtx[t] = (static_cast<uint32_t>(t) < ntx) ? a.offset + d : -1.0e30f;
-	}
+		for (int t = 0; t < NTXMAX; ++t) {
+			const uint32_t tx = blk * NTXMAX + static_cast<uint32_t>(t);
+			const uint32_t tt = min(tx, ntx - 1);   // unconditional load, no branch
+			const float d = a.delay[static_cast<uint64_t>(a.first_tx + tt) * npix + p];
+			dtx[t] = (tx < ntx) ? a.offset + d : -1.0e30f;
+		}
+	};
+	load_dtx(0);
 
 	// Issue the copies of chunk i into buffer b, given its window parameters
 	// pp.  Slot sl of pair j is copy number j * S + sl of the chunk and goes
 	// to wave (j * S + sl) % NW.
 	const uint32_t nchunks = nrx * CPR;
 	auto issue = [&](uint32_t i, uint32_t b, const WinParam* pp) {
-		const uint32_t r = i / CPR, c = i % CPR;
+		const uint32_t r = i / CPR, c = i % CPR;   // c == 0: first chunk of rx r
 		const uint32_t bbase = lds0 + b * buf_bytes;
 #pragma unroll
 		for (uint32_t j = 0; j < TC; ++j) {
@@ -301,7 +308,7 @@ das_tile_kernel(const TileArgs A)
 	// copies' on every chunk.
 	WinParam nx[TC];
 	auto load_params = [&](uint32_t k) {
-		const cu2_t pr = prm + (k / CPR) * NTXMAX + (k % CPR) * TC;
+		const cu2_t pr = prm + (k / CPR) * A.ntx_pad + (k % CPR) * TC;
 #pragma unroll
 		for (uint32_t j = 0; j < TC; ++j) {
 			nx[j].start = pr[j].start;
@@ -319,56 +326,60 @@ das_tile_kernel(const TileArgs A)
 	uint32_t i = 0, b = 0;
 	for (uint32_t r = 0; r < nrx; ++r) {
 		float drx = 0.0f, ar = 0.0f, ai = 0.0f;
+		for (uint32_t blk = 0; blk < nblk; ++blk) {
+			if (MULTI) load_dtx(blk);   // waited for at its first use
 #pragma clang loop unroll(full)
-		for (int c = 0; c < static_cast<int>(CPR); ++c, ++i) {
-			// copies issued after chunk i: chunks i+1 .. min(i+NB-2, last),
-			// QW each plus one for every chunk that starts an rx
-			const uint32_t hi = min(i + NB - 2, nchunks - 1);
-			const uint32_t after = (hi > i) ? (hi - i) * QW + (hi / CPR - i / CPR) : 0u;
-			wait_vmcnt_at_most(after);         // this wave's copies of chunk i landed
-			__syncthreads();                   // everyone's did; buffer of chunk i-1 is free
-			if (i + NB - 1 < nchunks) issue(i + NB - 1, (b + NB - 1) % NB, nx);
-			const uint32_t bbase = b * buf_bytes;
-			if (c == 0) drx = *reinterpret_cast<const float*>(smem + bbase + win_bytes + tid * 4);
-			// Register fences: this chunk's positions are computed after the
-			// barrier, and its accumulation is finished before the next one,
-			// so only one chunk's temporaries are ever live.
-			asm volatile("" : "+v"(drx));
-			const cu2_t pr = prm + r * NTXMAX + c * TC;
-			const cf32_t wr = (cf32_t)(a.w + static_cast<uint64_t>(r) * NTXMAX + c * TC);
+			for (int c = 0; c < static_cast<int>(CPB); ++c, ++i) {
+				// copies issued after chunk i: chunks i+1 .. min(i+NB-2, last),
+				// QW each plus one for every chunk that starts an rx
+				const uint32_t hi = min(i + NB - 2, nchunks - 1);
+				const uint32_t after = (hi > i) ? (hi - i) * QW + (hi / CPR - i / CPR) : 0u;
+				wait_vmcnt_at_most(after);         // this wave's copies of chunk i landed
+				__syncthreads();                   // everyone's did; buffer of chunk i-1 is free
+				if (i + NB - 1 < nchunks) issue(i + NB - 1, (b + NB - 1) % NB, nx);
+				const uint32_t bbase = b * buf_bytes;
+				if (c == 0 && blk == 0) drx = *reinterpret_cast<const float*>(smem + bbase + win_bytes + tid * 4);
+				// Register fences: this chunk's positions are computed after
+				// the barrier, and its accumulation is finished before the
+				// next one, so only one chunk's temporaries are ever live.
+				asm volatile("" : "+v"(drx));
+				const uint32_t t0 = blk * NTXMAX + c * TC;   // first transmit of the chunk
+				const cu2_t pr = prm + r * A.ntx_pad + t0;
+				const cf32_t wr = (cf32_t)(a.w + static_cast<uint64_t>(r) * a.w_stride + t0);
 #pragma unroll
-			for (int j = 0; j < TC; ++j) {
-				// One form for every pair: lanes outside 0 <= pos < L-1 get
-				// weight 0; the entry index is clamped into the window, which
-				// changes nothing for in-range lanes (floor(pos) - ws lies in
-				// [0, need - 2] by the window construction) and keeps excluded
-				// lanes' reads inside the buffer.
-				const int ws = static_cast<int>(pr[j].word & kWsMask);
-				const float pos = dtx[c * TC + j] + drx;
-				const float k = __builtin_amdgcn_fractf(pos);
-				const bool ok = pos >= 0.0f && pos < lim;
-				// load the weight unconditionally (a select, not a branch
-				// around a scalar load)
-				float wv = wr[j];
-				asm volatile("" : "+s"(wv));
-				const float w = ok ? wv : 0.0f;
-				const float pc = __builtin_amdgcn_fmed3f(pos, 0.0f, lim);   // NaN-free, no int overflow
-				const int rel = min(max(static_cast<int>(pc) - ws, 0), static_cast<int>(wcap) - 2);
-				const int off = rel * static_cast<int>(ESZ) + static_cast<int>(bbase + j * wcap * ESZ);
-				const E* e = reinterpret_cast<const E*>(smem + off);
-				const float2 v0 = Entry<STORAGE>::get(e[0]);
-				const float2 v1 = Entry<STORAGE>::get(e[1]);
-				ar = fmaf(fmaf(k, v1.x - v0.x, v0.x), w, ar);
-				ai = fmaf(fmaf(k, v1.y - v0.y, v0.y), w, ai);
+				for (int j = 0; j < TC; ++j) {
+					// One form for every pair: lanes outside 0 <= pos < L-1 get
+					// weight 0; the entry index is clamped into the window,
+					// which changes nothing for in-range lanes (floor(pos) - ws
+					// lies in [0, need - 2] by the window construction) and
+					// keeps excluded lanes' reads inside the buffer.
+					const int ws = static_cast<int>(pr[j].word & kWsMask);
+					const float pos = dtx[c * TC + j] + drx;
+					const float k = __builtin_amdgcn_fractf(pos);
+					const bool ok = pos >= 0.0f && pos < lim;
+					// load the weight unconditionally (a select, not a branch
+					// around a scalar load)
+					float wv = wr[j];
+					asm volatile("" : "+s"(wv));
+					const float w = ok ? wv : 0.0f;
+					const float pc = __builtin_amdgcn_fmed3f(pos, 0.0f, lim);   // NaN-free, no int overflow
+					const int rel = min(max(static_cast<int>(pc) - ws, 0), static_cast<int>(wcap) - 2);
+					const int off = rel * static_cast<int>(ESZ) + static_cast<int>(bbase + j * wcap * ESZ);
+					const E* e = reinterpret_cast<const E*>(smem + off);
+					const float2 v0 = Entry<STORAGE>::get(e[0]);
+					const float2 v1 = Entry<STORAGE>::get(e[1]);
+					ar = fmaf(fmaf(k, v1.x - v0.x, v0.x), w, ar);
+					ai = fmaf(fmaf(k, v1.y - v0.y, v0.y), w, ai);
+				}
+				asm volatile("" : "+v"(ar), "+v"(ai));
+				// after this chunk's LDS reads (a pending scalar load would
+				// make every LDS wait a full drain); the chunk index passes
+				// through a fence so the loads stay here
+				uint32_t nk = i + NB;
+				asm volatile("" : "+s"(nk));
+				if (nk < nchunks) load_params(nk);
+				b = (b + 1 == NB) ? 0u : b + 1;
 			}
-			asm volatile("" : "+v"(ar), "+v"(ai));
-			// after this chunk's LDS reads (a pending scalar load would make
-			// every LDS wait a full drain); the chunk index passes through a
-			// fence so the loads stay here
-			uint32_t nk = i + NB;
-			asm volatile("" : "+s"(nk));
-			if (nk < nchunks) load_params(nk);
-			b = (b + 1 == NB) ? 0u : b + 1;
 		}
 		st.add_rx(ar, ai, r == 0);
 	}
@@ -393,9 +404,15 @@ void launch_tile_t(const TileArgs& A, uint32_t ntiles, uint32_t tc, hipStream_t 
 				static_cast<int>(lds)));
 		hipLaunchKernelGGL(kern, dim3(ntiles), dim3(threads), lds, s, A);
 	};
-	if (nt == 512) go(das_tile_kernel<STORAGE, CF, NTXMAX, 8, 512>, 512);
-	else if (tc == 8) go(das_tile_kernel<STORAGE, CF, NTXMAX, 8, 256>, 256);
-	else go(das_tile_kernel<STORAGE, CF, NTXMAX, 4, 256>, 256);
+	if constexpr (NTXMAX == 128) {
+		if (A.ntx_pad > NTXMAX) {   // several transmit blocks: tc = 8, nt = 256
+			go(das_tile_kernel<STORAGE, CF, NTXMAX, 8, 256, true>, 256);
+			return;
+		}
+	}
+	if (nt == 512) go(das_tile_kernel<STORAGE, CF, NTXMAX, 8, 512, false>, 512);
+	else if (tc == 8) go(das_tile_kernel<STORAGE, CF, NTXMAX, 8, 256, false>, 256);
+	else go(das_tile_kernel<STORAGE, CF, NTXMAX, 4, 256, false>, 256);
 }
 
 template<int STORAGE, int CF>
