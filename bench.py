@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--workload", default="sta2d_128_pcf")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--kernel", type=int, default=0, help="DAS variant (0 auto, 1 gather, 2 tiled)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay steady frames as one hipGraph (stage times then cover the whole frame)")
     ap.add_argument("--tile", type=int, nargs=2, metavar=("SHAPE", "CHUNKING"), default=None,
                     help="fixed tiled-DAS parameters instead of the autotuner (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -270,6 +272,8 @@ def main():
         proc.set_kernel(2, *args.tile)
     elif args.kernel:
         proc.set_kernel(args.kernel)
+    if args.graph:
+        proc.set_graph(True)
     d_value = torch.empty(npix, dtype=torch.float32, device=dev)
     d_factor = torch.empty(npix, dtype=torch.float32, device=dev)
 
@@ -320,7 +324,8 @@ def main():
         "config": {"workload": wl.name, "elements": wl.num_elements, "tx": wl.num_tx, "samples": wl.num_samples,
                    "upsampling": wl.upsampling, "grid": [wl.n_x, wl.n_z], "coherence": wl.cf,
                    "storage": args.storage, "pixel_pairs_per_frame": wl.pixel_pairs,
-                   "das_tile": tile_name(tuple(args.tile) if args.tile else proc.tuned_kernel())},
+                   "das_tile": tile_name(tuple(args.tile) if args.tile else proc.tuned_kernel()),
+                   "graph": bool(args.graph)},
         "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                              "p99": round(float(np.percentile(lat, 99)), 4)},
         "stage_ms": {"prep": round(prep_avg, 4), "das": round(das_avg, 4)},
@@ -328,6 +333,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": traffic_src,
+                     # measured HBM-side bytes per launch over this run's launch time
+                     "traffic_gbs": round(traffic / (das_avg * 1e-3) / 1e9, 1) if traffic else None,
                      "kernel": "das (delay-and-sum + PCF + envelope)", "algorithmic_bytes": alg},
     }
     if not args.no_cpu_baseline:

@@ -201,6 +201,18 @@ int mista_query(const mista_handle* h, uint32_t* signal_length, uint32_t* fft_si
  * one-pixel-per-lane gather, 2 = LDS-tiled).  Tile parameters 0 = default. */
 int mista_set_kernel(mista_handle* h, int variant, int tile_param0, int tile_param1);
 
+/* Frame graphs for mista_process_device (no counterpart in the reference:
+ * for callers that replay the same frame shape).  With enable != 0, a device
+ * frame that changes nothing the kernels depend on (same echo/output
+ * pointers and sample count, no new allocation, table, tuning or setting) is
+ * recorded once as a hipGraph on the handle's stream, and later frames with
+ * the same key replay it with one hipGraphLaunch.  Only the fused
+ * preparation path is captured.  In graph frames mista_last_timing reports
+ * the whole frame as DAS time.  mista_graph_stats counts captures and
+ * replays. */
+int mista_set_graph(mista_handle* h, int enable);
+int mista_graph_stats(const mista_handle* h, uint64_t* captures, uint64_t* launches);
+
 /* Tile parameters the automatic variant chose on its first frame of the
  * current geometry (it times each tile shape and LDS budget once); -1 in
  * tile_param0 when nothing was tuned or the gather kernel runs.  No

@@ -142,7 +142,14 @@ def test_cli_project_config_matches_oracle(tmp_path):
     f = np.fromfile(str(out) + "_factor.f32", np.float32).reshape(n1, n2)
     xz = np.stack([np.fromfile(dump / "grid_x.f64").astype(np.float32),
                    np.fromfile(dump / "grid_z.f64").astype(np.float32)], axis=-1).reshape(n1, n2, 2)
-    v_o, f_o, _, _ = oracle.process(wl.oracle_cfg(), echoes, xz, wl.rx_apodization())
+    cfg_o = wl.oracle_cfg()
+    v_o, f_o, sums = oracle.das_rxsums(cfg_o, oracle.prepare(cfg_o, echoes, np.float32), xz, wl.rx_apodization())
     peak = float(np.abs(v_o).max())
     assert float(np.abs(v - v_o).max()) / peak <= 1e-4
-    assert float(np.abs(v * f - v_o * f_o).max()) / peak <= 1e-4
+    # CF image to the same bar except where the oracle's per-rx sums sit on
+    # the PCF's atan2 cut (any two FP32 summation orders may disagree there)
+    bad = np.abs(v.astype(np.float64) * f - v_o.astype(np.float64) * f_o) > 1e-4 * peak
+    sens = oracle.branch_sensitive(sums, oracle.CF_PCF)
+    print("[gpu-test] CF image over the bar at", int(bad.sum()), "pixels; unexplained", int((bad & ~sens).sum()),
+          flush=True)
+    assert not np.any(bad & ~sens) and bad.mean() <= 0.01

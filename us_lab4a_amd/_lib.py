@@ -186,6 +186,8 @@ SIGNATURES = {
     "mista_query": (_i32, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u64)]),
     "mista_set_kernel": (_i32, [_vp, _i32, _i32, _i32]),
     "mista_tuned_kernel": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "mista_set_graph": (_i32, [_vp, _i32]),
+    "mista_graph_stats": (_i32, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
     "mista_last_timing": (_i32, [_vp, _fp, _fp]),
     "mista_synth_echoes_device": (_i32, [ctypes.POINTER(SynthParams), _vp, _vp, _vp, _vp, _vp]),
     "mista_sum_squares_device": (_i32, [_vp, _u64, ctypes.POINTER(ctypes.c_double), _vp]),

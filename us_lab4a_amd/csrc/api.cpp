@@ -60,8 +60,37 @@ struct mista_handle {
 	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 	bool timed = false;
 
+	// frame graph (mista_set_graph): the kernels of a steady device frame,
+	// captured once and replayed while nothing they depend on changes
+	bool graph_on = false;
+	bool capturing = false;        // launch-only frame path: no memset, no events
+	uint64_t settings_epoch = 0;   // bumped by every setter
+	struct GraphKey {
+		const void* echoes = nullptr;
+		const void* value = nullptr;
+		const void* factor = nullptr;
+		uint32_t ns = 0;
+		uint64_t settings = 0, allocs = 0, grid = 0, tuned = 0;
+		int p0 = 0, p1 = 0;
+		bool operator==(const GraphKey& o) const
+		{
+			return echoes == o.echoes && value == o.value && factor == o.factor && ns == o.ns &&
+					settings == o.settings && allocs == o.allocs && grid == o.grid && tuned == o.tuned &&
+					p0 == o.p0 && p1 == o.p1;
+		}
+	} gkey;
+	hipGraphExec_t gexec = nullptr;
+	uint64_t graph_launches = 0, graph_captures = 0;
+
+	void drop_graph()
+	{
+		if (gexec) (void)hipGraphExecDestroy(gexec);
+		gexec = nullptr;
+	}
+
 	~mista_handle()
 	{
+		drop_graph();
 		for (auto& e : ev)
 			if (e) (void)hipEventDestroy(e);
 		if (stream) (void)hipStreamDestroy(stream);
@@ -162,8 +191,9 @@ void set_frame_geom(mista_handle* h, uint32_t ns, uint32_t L, hipStream_t s)
 	const size_t nch = static_cast<size_t>(g.ntx) * g.nrx;
 	const size_t tbytes = nch * L * storage_bytes(c.storage);
 	unsigned char* t = static_cast<unsigned char*>(h->d_tensor.reserve(tbytes + tensor_tail_padding()));
-	// the staged-window copies may read into the padding: keep it zero
-	HIP_CHECK(hipMemsetAsync(t + tbytes, 0, tensor_tail_padding(), s));
+	// the staged-window copies may read into the padding: keep it zero (a
+	// captured frame graph reuses a tail that an eager frame has cleared)
+	if (!h->capturing) HIP_CHECK(hipMemsetAsync(t + tbytes, 0, tensor_tail_padding(), s));
 	h->d_scale.reserve(nch * sizeof(float));
 	h->d_w.reserve(static_cast<size_t>(g.ntx_pad) * g.nrx * sizeof(float));
 }
@@ -224,14 +254,14 @@ void run_prepare(mista_handle* h, const float* d_echoes, uint32_t ns, hipStream_
 	set_frame_geom(h, ns, static_cast<uint32_t>(L64), s);
 	ensure_filter(h, s);
 	ensure_events(h);
-	HIP_CHECK(hipEventRecord(h->ev[0], s));
+	if (!h->capturing) HIP_CHECK(hipEventRecord(h->ev[0], s));
 	if (h->rf)
 		prepare_signals_rf(h->geom, d_echoes, h->d_hpoly.as<const float>(), h->filter_J, h->filter_off,
 				h->d_tensor.as<float>(), s);
 	else
 		prepare_signals(h->prep, h->geom, d_echoes, h->d_hpoly.as<const float>(), h->filter_J, h->filter_off,
 				h->d_tensor.get(), h->d_scale.as<float>(), s);
-	HIP_CHECK(hipEventRecord(h->ev[1], s));
+	if (!h->capturing) HIP_CHECK(hipEventRecord(h->ev[1], s));
 	h->signal_ready = true;
 }
 
@@ -250,10 +280,10 @@ void run_das(mista_handle* h, float* d_value, float* d_factor, hipStream_t s)
 	// w[tx][rx] = rxApod * txApod * per-channel storage scale
 	launch_weights(h->geom, h->d_rx_apod.as<const float>(), h->tx_apod_set ? h->d_tx_apod.as<const float>() : nullptr,
 			h->d_scale.as<const float>(), h->d_w.as<float>(), s);
-	HIP_CHECK(hipEventRecord(h->ev[2], s));
+	if (!h->capturing) HIP_CHECK(hipEventRecord(h->ev[2], s));
 	launch_das(h->geom, h->tuning, h->das_scratch, h->d_tensor.get(), h->d_delay.as<const float>(), h->d_w.as<const float>(),
 			d_value, d_factor, s);
-	HIP_CHECK(hipEventRecord(h->ev[3], s));
+	if (!h->capturing) HIP_CHECK(hipEventRecord(h->ev[3], s));
 	h->timed = true;
 }
 
@@ -495,6 +525,7 @@ int mista_set_elements_3d(mista_handle* h, const float* tx_xy, const float* rx_x
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
 	return guarded(h, [&] {
+		++h->settings_epoch;
 		if (!h->is3d) MISTA_THROW(MISTA_E_INVALID_STATE, "Not a 3-D processor.");
 		if (!tx_xy || !rx_xy) MISTA_THROW(MISTA_E_INVALID_PARAM, "null argument.");
 		set_elements_3d(h, tx_xy, rx_xy);
@@ -522,6 +553,8 @@ int mista_set_rx_apod(mista_handle* h, const float* rx_apod, uint32_t n)
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
 	return guarded(h, [&] {
+		++h->settings_epoch;
+		++h->settings_epoch;
 		if (!rx_apod) MISTA_THROW(MISTA_E_INVALID_PARAM, "rx_apod is null.");
 		if (n != h->cfg.num_elements)
 			MISTA_THROW(MISTA_E_INVALID_VALUE, "Wrong rx apodization size: " << n << " (should be " << h->cfg.num_elements << ").");
@@ -534,6 +567,7 @@ int mista_set_tx_apod(mista_handle* h, const float* tx_apod, uint32_t n)
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
 	return guarded(h, [&] {
+		++h->settings_epoch;
 		if (!tx_apod) {
 			h->tx_apod.clear();
 			h->tx_apod_set = false;
@@ -551,6 +585,7 @@ int mista_set_grid(mista_handle* h, const float* grid_xz, uint32_t n1, uint32_t 
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
 	return guarded(h, [&] {
+		++h->settings_epoch;
 		if (!grid_xz || n1 == 0 || n2 == 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "Empty image grid.");
 		const uint64_t npix = static_cast<uint64_t>(n1) * n2;
 		std::vector<float> xyz(npix * 3);
@@ -567,6 +602,7 @@ int mista_set_grid_xyz(mista_handle* h, const float* grid_xyz, uint32_t n1, uint
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
 	return guarded(h, [&] {
+		++h->settings_epoch;
 		if (!grid_xyz || n1 == 0 || n2 == 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "Empty image grid.");
 		const uint64_t npix = static_cast<uint64_t>(n1) * n2;
 		set_grid_xyz(h, std::vector<float>(grid_xyz, grid_xyz + 3 * npix), n1, n2, ny);
@@ -577,6 +613,7 @@ int mista_set_grid_points(mista_handle* h, const float* points, uint32_t n1, uin
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
 	return guarded(h, [&] {
+		++h->settings_epoch;
 		if (!points || n1 == 0 || n2 == 0) MISTA_THROW(MISTA_E_INVALID_PARAM, "Empty image grid.");
 		const uint64_t npix = static_cast<uint64_t>(n1) * n2;
 		std::vector<float> xyz(npix * 3);
@@ -593,9 +630,86 @@ int mista_process_device(mista_handle* h, const float* d_echoes, uint32_t num_sa
 	return guarded(h, [&] {
 		if (!d_echoes || !d_value) MISTA_THROW(MISTA_E_INVALID_PARAM, "null device pointer.");
 		hipStream_t s = pick_stream(h, stream);
+		if (!h->graph_on) {
+			run_prepare(h, d_echoes, num_samples, s);
+			run_das(h, d_value, d_factor, s);
+			return;
+		}
+		auto key = [&] {
+			mista_handle::GraphKey k;
+			k.echoes = d_echoes;
+			k.value = d_value;
+			k.factor = d_factor;
+			k.ns = num_samples;
+			k.settings = h->settings_epoch;
+			k.allocs = alloc_epoch().load();
+			k.grid = h->grid_version;
+			k.tuned = h->das_scratch.tuned_key ^ h->das_scratch.params_key;
+			k.p0 = h->das_scratch.tuned_p0;
+			k.p1 = h->das_scratch.tuned_p1;
+			return k;
+		};
+		if (h->gexec && key() == h->gkey) {
+			ensure_events(h);
+			HIP_CHECK(hipEventRecord(h->ev[0], s));
+			HIP_CHECK(hipEventRecord(h->ev[1], s));
+			HIP_CHECK(hipEventRecord(h->ev[2], s));
+			HIP_CHECK(hipGraphLaunch(h->gexec, s));
+			HIP_CHECK(hipEventRecord(h->ev[3], s));
+			h->timed = true;
+			++h->graph_launches;
+			return;
+		}
+		// eager frame; if it changed nothing the kernels depend on (no
+		// allocation, table build, tuning or plan), record the same launches
+		// as a graph on the handle's stream for the frames that follow
+		h->drop_graph();
+		const mista_handle::GraphKey before = key();
+		const bool plan_ready = h->prep.plan != nullptr;
 		run_prepare(h, d_echoes, num_samples, s);
 		run_das(h, d_value, d_factor, s);
+		// (the library-FFT preparation and the RF path stay eager)
+		if (!(key() == before) || !plan_ready || h->rf || !h->prep.last_fused) return;
+		hipGraph_t graph = nullptr;
+		HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+		h->capturing = true;
+		try {
+			run_prepare(h, d_echoes, num_samples, h->stream);
+			run_das(h, d_value, d_factor, h->stream);
+		} catch (...) {
+			h->capturing = false;
+			(void)hipStreamEndCapture(h->stream, &graph);
+			if (graph) (void)hipGraphDestroy(graph);
+			throw;
+		}
+		h->capturing = false;
+		HIP_CHECK(hipStreamEndCapture(h->stream, &graph));
+		const hipError_t ie = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
+		(void)hipGraphDestroy(graph);
+		if (ie != hipSuccess) {
+			h->gexec = nullptr;
+			MISTA_THROW(MISTA_E_HIP, "hipGraphInstantiate failed: " << hipGetErrorString(ie));
+		}
+		h->gkey = key();
+		++h->graph_captures;
 	});
+}
+
+int mista_set_graph(mista_handle* h, int enable)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		h->graph_on = enable != 0;
+		h->drop_graph();
+	});
+}
+
+int mista_graph_stats(const mista_handle* h, uint64_t* captures, uint64_t* launches)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	if (captures) *captures = h->graph_captures;
+	if (launches) *launches = h->graph_launches;
+	return MISTA_OK;
 }
 
 int mista_prepare_signals_device(mista_handle* h, const float* d_echoes, uint32_t num_samples, void* stream)
@@ -656,6 +770,7 @@ int mista_upload_analytic(mista_handle* h, const float* analytic_ri, uint32_t si
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
 	return guarded(h, [&] {
+		++h->settings_epoch;
 		if (!analytic_ri) MISTA_THROW(MISTA_E_INVALID_PARAM, "null host pointer.");
 		require_grid(h);
 		hipStream_t s = h->stream;
@@ -725,6 +840,7 @@ int mista_set_kernel(mista_handle* h, int variant, int p0, int p1)
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
 	return guarded(h, [&] {
+		++h->settings_epoch;
 		if (variant < 0 || variant > 2) MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid kernel variant " << variant);
 		h->tuning.variant = variant;
 		h->tuning.p0 = p0;
@@ -744,6 +860,7 @@ int mista_set_prep(mista_handle* h, int method)
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
 	return guarded(h, [&] {
+		++h->settings_epoch;
 		if (method < 0 || method > 2) MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid preparation method " << method);
 		h->prep.method = method;
 	});

@@ -5,6 +5,7 @@
 #include <hip/hip_fp16.h>
 #include <hipfft/hipfft.h>
 
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -33,6 +34,14 @@ struct Error : std::runtime_error {
 	MISTA_THROW(MISTA_E_HIP, #expr << ": hipfft error " << static_cast<int>(r_) << " (" << __FILE__ << ":" << __LINE__ << ")"); } while (0)
 
 // Device buffer owned by a handle.  Reallocates only when growing.
+// Count of device allocations made by DeviceBuffer (process-wide): a change
+// tells a captured frame graph that a buffer it uses may have moved.
+inline std::atomic<uint64_t>& alloc_epoch()
+{
+	static std::atomic<uint64_t> e{0};
+	return e;
+}
+
 class DeviceBuffer {
 public:
 	DeviceBuffer() = default;
@@ -50,6 +59,7 @@ public:
 				MISTA_THROW(MISTA_E_NO_MEMORY, "hipMalloc(" << bytes << " bytes) failed: " << hipGetErrorString(e));
 			}
 			cap_ = bytes;
+			++alloc_epoch();
 		}
 		return ptr_;
 	}
@@ -129,6 +139,7 @@ struct PrepPlan;  // hipFFT plans + scratch for signal preparation
 struct PrepState {
 	PrepPlan* plan = nullptr;  // owned; destroyed in prep.hip where PrepPlan is complete
 	int method = 0;            // 0 automatic, 1 library FFT (hipFFT), 2 fused LDS kernel
+	bool last_fused = false;   // the last preparation ran the fused kernel
 	PrepState() = default;
 	PrepState(const PrepState&) = delete;
 	PrepState& operator=(const PrepState&) = delete;

@@ -903,10 +903,12 @@ void prepare_signals(PrepState& st, const FrameGeom& g, const float* d_raw, cons
 		const size_t lds = fused_lds_bytes(g, J, off);
 		if (lds) {
 			prepare_fused(st, g, d_raw, d_filter_poly, J, off, d_tensor, d_scale, lds, s);
+			st.last_fused = true;
 			return;
 		}
 		if (st.method == 2) MISTA_THROW(MISTA_E_INVALID_PARAM, "the fused preparation does not fit this frame (FFT size " << g.nf << ")");
 	}
+	st.last_fused = false;
 	const uint32_t chunk = std::min<uint32_t>(nch, 1024u);
 	if (!st.plan || !st.plan->r2c || st.plan->nf != g.nf || st.plan->chunk != chunk || st.plan->tail != nch % chunk) {
 		delete st.plan;

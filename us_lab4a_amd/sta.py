@@ -158,6 +158,16 @@ class HipSTAProcessor:
     def set_kernel(self, variant, p0=0, p1=0):
         check(lib().mista_set_kernel(self._h, variant, p0, p1), self._h)
 
+    def set_graph(self, enable=True):
+        """Replay steady device frames (process_device) as one hipGraph."""
+        check(lib().mista_set_graph(self._h, 1 if enable else 0), self._h)
+
+    def graph_stats(self):
+        """(captures, replays) of the frame graph."""
+        a, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        check(lib().mista_graph_stats(self._h, ctypes.byref(a), ctypes.byref(b)), self._h)
+        return a.value, b.value
+
     def tuned_kernel(self):
         """(tile shape, chunking) the automatic variant chose; (-1, 0) when
         nothing was tuned or the gather kernel runs."""
