@@ -354,7 +354,8 @@ def tile_name(tuned):
         return "gather kernel"
     shape = {2: "8x32", 3: "32x8", 4: "8x8x4", 5: "16x32", 6: "32x16", 7: "8x8x8"}.get(p0, "16x16")
     kb = (p1 >> 16) & 0xFF
-    return f"{shape} tile, {kb or 'default'} KiB LDS ring"
+    tc = p1 & 0xFF
+    return f"{shape} tile, {kb or 'default'} KiB LDS ring, {tc or 'default'} pairs per chunk"
 
 
 if __name__ == "__main__":

@@ -333,9 +333,10 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 			if (launch_tiled(g, t.p0, t.p1, scratch, a, d_delay, s)) return;
 		} else {
 			// Autotuner: on the first frame of a geometry, time every
-			// candidate tile shape and LDS budget (each writes the same
-			// images) and keep the fastest for the frames that follow.  The
-			// first run of a shape builds its window tables and is not timed.
+			// candidate tile shape, LDS budget and chunk size (each writes
+			// the same images) and keep the fastest for the frames that
+			// follow.  The first run of a shape builds its window tables and
+			// is not timed.
 			const uint64_t key = g.grid_version ^ (static_cast<uint64_t>(g.L) << 20) ^ (static_cast<uint64_t>(g.ntx) << 40) ^
 					(static_cast<uint64_t>(g.nrx) << 50) ^ (static_cast<uint64_t>(g.storage) << 60) ^
 					(static_cast<uint64_t>(g.cf_kind) << 62) ^ static_cast<uint64_t>(g.offset * 1024.0f);
@@ -353,19 +354,21 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 					if (!launch_tiled(g, p0, kbs[0] << 16, scratch, a, d_delay, s) &&
 							!launch_tiled(g, p0, kbs[1] << 16, scratch, a, d_delay, s))
 						continue;
-					for (int kb : kbs) {
-						HIP_CHECK(hipEventRecord(e0, s));
-						const bool ok = launch_tiled(g, p0, kb << 16, scratch, a, d_delay, s);
-						HIP_CHECK(hipEventRecord(e1, s));
-						HIP_CHECK(hipEventSynchronize(e1));
-						float ms = 0.0f;
-						HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-						if (ok && (bp0 < 0 || ms < best)) {
-							best = ms;
-							bp0 = p0;
-							bp1 = kb << 16;
+					for (int kb : kbs)
+						for (int tc : {4, 8}) {   // pairs per chunk
+							const int p1 = tc | (kb << 16);
+							HIP_CHECK(hipEventRecord(e0, s));
+							const bool ok = launch_tiled(g, p0, p1, scratch, a, d_delay, s);
+							HIP_CHECK(hipEventRecord(e1, s));
+							HIP_CHECK(hipEventSynchronize(e1));
+							float ms = 0.0f;
+							HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+							if (ok && (bp0 < 0 || ms < best)) {
+								best = ms;
+								bp0 = p0;
+								bp1 = p1;
+							}
 						}
-					}
 				}
 				(void)hipEventDestroy(e0);
 				(void)hipEventDestroy(e1);
