@@ -80,6 +80,8 @@ struct mista_handle {
 		}
 	} gkey;
 	hipGraphExec_t gexec = nullptr;
+	GraphKey gfail;                // key whose recording failed
+	bool gfail_valid = false;
 	uint64_t graph_launches = 0, graph_captures = 0;
 
 	void drop_graph()
@@ -670,25 +672,32 @@ int mista_process_device(mista_handle* h, const float* d_echoes, uint32_t num_sa
 		run_das(h, d_value, d_factor, s);
 		// (the library-FFT preparation and the RF path stay eager)
 		if (!(key() == before) || !plan_ready || h->rf || !h->prep.last_fused) return;
+		// The frame is done; a recording that fails leaves the next frames
+		// eager (the same key is not tried again) and is not an error.
+		if (h->gfail_valid && h->gfail == before) return;
 		hipGraph_t graph = nullptr;
-		HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-		h->capturing = true;
-		try {
-			run_prepare(h, d_echoes, num_samples, h->stream);
-			run_das(h, d_value, d_factor, h->stream);
-		} catch (...) {
+		bool ok = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+		if (ok) {
+			h->capturing = true;
+			try {
+				run_prepare(h, d_echoes, num_samples, h->stream);
+				run_das(h, d_value, d_factor, h->stream);
+			} catch (...) {
+				ok = false;
+			}
 			h->capturing = false;
-			(void)hipStreamEndCapture(h->stream, &graph);
-			if (graph) (void)hipGraphDestroy(graph);
-			throw;
+			ok = (hipStreamEndCapture(h->stream, &graph) == hipSuccess) && ok && graph;
 		}
-		h->capturing = false;
-		HIP_CHECK(hipStreamEndCapture(h->stream, &graph));
-		const hipError_t ie = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
-		(void)hipGraphDestroy(graph);
-		if (ie != hipSuccess) {
+		if (ok && hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
 			h->gexec = nullptr;
-			MISTA_THROW(MISTA_E_HIP, "hipGraphInstantiate failed: " << hipGetErrorString(ie));
+			ok = false;
+		}
+		if (graph) (void)hipGraphDestroy(graph);
+		(void)hipGetLastError();   // a failed recording leaves no sticky error behind
+		if (!ok) {
+			h->gfail = before;
+			h->gfail_valid = true;
+			return;
 		}
 		h->gkey = key();
 		++h->graph_captures;

@@ -347,7 +347,11 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 				HIP_CHECK(hipEventCreate(&e1));
 				float best = 0.0f;
 				int bp0 = -1, bp1 = 0;
+				// frames this long gain little from tuning: the first fitting
+				// configuration is kept once a run takes longer than this
+				constexpr float kTuneLimitMs = 150.0f;
 				for (int p0 : shapes) {
+					if (bp0 >= 0 && best > kTuneLimitMs) break;
 					// 256-lane tiles: three or two workgroups per CU; 512-lane
 					// tiles: one
 					const int kbs[2] = {p0 >= 5 ? 76 : 52, p0 >= 5 ? 150 : 76};
@@ -356,6 +360,7 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 						continue;
 					for (int kb : kbs)
 						for (int tc : {4, 8}) {   // pairs per chunk
+							if (bp0 >= 0 && best > kTuneLimitMs) break;
 							const int p1 = tc | (kb << 16);
 							HIP_CHECK(hipEventRecord(e0, s));
 							const bool ok = launch_tiled(g, p0, p1, scratch, a, d_delay, s);
