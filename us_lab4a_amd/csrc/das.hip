@@ -359,7 +359,7 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 							!launch_tiled(g, p0, kbs[1] << 16, scratch, a, d_delay, s))
 						continue;
 					for (int kb : kbs)
-						for (int tc : {4, 8}) {   // pairs per chunk
+						for (int tc : {8, 4}) {   // pairs per chunk (8 first: the better one when it fits)
 							if (bp0 >= 0 && best > kTuneLimitMs) break;
 							const int p1 = tc | (kb << 16);
 							HIP_CHECK(hipEventRecord(e0, s));
