@@ -120,34 +120,39 @@ void launch_gather(const FrameGeom& g, const DasArgs& a, hipStream_t s)
 __global__ void __launch_bounds__(256) tile_bounds_kernel(const float* __restrict__ delay, uint64_t npix,
 		uint32_t num_elements, TileGeom tg, float2* __restrict__ bounds)
 {
-	__shared__ float rmin[4], rmax[4];
+	// Each wave takes every fourth element and reduces over the tile's
+	// points with shuffles (no workgroup barrier per element); the points'
+	// table offsets are computed once.  Lanes outside the grid read the point
+	// they shadow, as in the DAS kernels.
+	constexpr uint32_t kMaxPts = 512 / 64;
 	const uint32_t tile = blockIdx.x;
 	const uint32_t npt = tg.tw1 * tg.tw2 * tg.tzh;
-	for (uint32_t e = 0; e < num_elements; ++e) {
+	const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	uint64_t off[kMaxPts];
+#pragma unroll
+	for (uint32_t k = 0; k < kMaxPts; ++k) {
+		const uint32_t l = lane + 64 * k;
+		uint32_t col = 0, row = 0;
+		bool valid;
+		tg.pixel(tile, l < npt ? l : 0u, col, row, valid);
+		off[k] = static_cast<uint64_t>(col) * tg.n2 + row;
+	}
+	for (uint32_t e = wave; e < num_elements; e += 4) {
+		const float* de = delay + static_cast<uint64_t>(e) * npix;
 		float mn = 3.4e38f, mx = -3.4e38f;
-		for (uint32_t l = threadIdx.x; l < npt; l += 256) {
-			uint32_t col, row;
-			bool valid;
-			tg.pixel(tile, l, col, row, valid);
-			const float d = delay[static_cast<uint64_t>(e) * npix + static_cast<uint64_t>(col) * tg.n2 + row];
-			mn = fminf(mn, d);
-			mx = fmaxf(mx, d);
+#pragma unroll
+		for (uint32_t k = 0; k < kMaxPts; ++k) {
+			if (lane + 64 * k < npt) {
+				const float d = de[off[k]];
+				mn = fminf(mn, d);
+				mx = fmaxf(mx, d);
+			}
 		}
 		for (int o = 32; o > 0; o >>= 1) {
-			mn = fminf(mn, __shfl_down(mn, o, 64));
-			mx = fmaxf(mx, __shfl_down(mx, o, 64));
+			mn = fminf(mn, __shfl_xor(mn, o, 64));
+			mx = fmaxf(mx, __shfl_xor(mx, o, 64));
 		}
-		if ((threadIdx.x & 63) == 0) {
-			rmin[threadIdx.x >> 6] = mn;
-			rmax[threadIdx.x >> 6] = mx;
-		}
-		__syncthreads();
-		if (threadIdx.x == 0) {
-			bounds[static_cast<uint64_t>(tile) * num_elements + e] =
-					make_float2(fminf(fminf(rmin[0], rmin[1]), fminf(rmin[2], rmin[3])),
-							fmaxf(fmaxf(rmax[0], rmax[1]), fmaxf(rmax[2], rmax[3])));
-		}
-		__syncthreads();
+		if (lane == 0) bounds[static_cast<uint64_t>(tile) * num_elements + e] = make_float2(mn, mx);
 	}
 }
 
@@ -155,10 +160,13 @@ __global__ void __launch_bounds__(256) window_params_kernel(const float2* __rest
 		uint32_t num_elements, uint32_t first_tx, uint32_t ntx, uint32_t ntx_pad, uint32_t nrx, uint32_t L, float offset,
 		uint32_t slot, WinParam* __restrict__ params, uint32_t* __restrict__ max_slots)
 {
-	const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	// Grid-stride over the entries (a capped grid): each lane folds the
+	// slot counts of many entries, then one atomic per wave.
 	const uint64_t n = static_cast<uint64_t>(ntiles) * nrx * ntx_pad;
-	uint32_t slots = 0;   // every lane reaches the wave reduction below
-	if (i < n) {
+	uint32_t mslots = 0;   // every lane reaches the wave reduction below
+	for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+			i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+		uint32_t slots = 0;
 		const uint32_t t = static_cast<uint32_t>(i % ntx_pad);
 		const uint32_t r = static_cast<uint32_t>((i / ntx_pad) % nrx);
 		const uint32_t tile = static_cast<uint32_t>(i / (static_cast<uint64_t>(ntx_pad) * nrx));
@@ -185,10 +193,10 @@ __global__ void __launch_bounds__(256) window_params_kernel(const float2* __rest
 			}
 		}
 		params[i] = WinParam{start, word};
+		mslots = max(mslots, slots);
 	}
-	// one atomic per wave
-	for (int o = 32; o > 0; o >>= 1) slots = max(slots, static_cast<uint32_t>(__shfl_xor(static_cast<int>(slots), o, 64)));
-	if ((threadIdx.x & 63) == 0 && slots) atomicMax(max_slots, slots);
+	for (int o = 32; o > 0; o >>= 1) mslots = max(mslots, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mslots), o, 64)));
+	if ((threadIdx.x & 63) == 0 && mslots) atomicMax(max_slots, mslots);
 }
 
 }  // namespace
@@ -235,7 +243,8 @@ bool launch_tiled(const FrameGeom& g, int p0, int p1, DasScratch& scratch, const
 		WinParam* prm = static_cast<WinParam*>(scratch.params.reserve(np * sizeof(WinParam)));
 		uint32_t* d_max = static_cast<uint32_t*>(scratch.partial.reserve(256));
 		HIP_CHECK(hipMemsetAsync(d_max, 0, sizeof(uint32_t), s));
-		hipLaunchKernelGGL(window_params_kernel, dim3(static_cast<uint32_t>((np + 255) / 256)), dim3(256), 0, s, b, ntiles,
+		hipLaunchKernelGGL(window_params_kernel, dim3(static_cast<uint32_t>(std::min<uint64_t>((np + 255) / 256, 4096))),
+				dim3(256), 0, s, b, ntiles,
 				g.num_elements, g.first_tx, g.ntx, g.ntx_pad, g.nrx, g.L, g.offset, slot, prm, d_max);
 		HIP_CHECK(hipGetLastError());
 		uint32_t max_slots = 0;
@@ -350,25 +359,45 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 				// frames this long gain little from tuning: the first fitting
 				// configuration is kept once a run takes longer than this
 				constexpr float kTuneLimitMs = 150.0f;
+				int last_p0 = -1;   // shape whose tables are resident
 				for (int p0 : shapes) {
 					if (bp0 >= 0 && best > kTuneLimitMs) break;
 					// 256-lane tiles: three or two workgroups per CU; 512-lane
 					// tiles: one
 					const int kbs[2] = {p0 >= 5 ? 76 : 52, p0 >= 5 ? 150 : 76};
-					if (!launch_tiled(g, p0, kbs[0] << 16, scratch, a, d_delay, s) &&
-							!launch_tiled(g, p0, kbs[1] << 16, scratch, a, d_delay, s))
-						continue;
+					// untimed for comparison (it builds the shape's tables), but
+					// a first run this long ends the search with its own result
+					HIP_CHECK(hipEventRecord(e0, s));
+					const int tries[3] = {8 | (kbs[0] << 16), 4 | (kbs[0] << 16), kbs[1] << 16};
+					int built = 0;
+					bool ok = false;
+					for (int k = 0; k < 3 && !ok; ++k) {
+						built = tries[k];
+						ok = launch_tiled(g, p0, built, scratch, a, d_delay, s);
+					}
+					HIP_CHECK(hipEventRecord(e1, s));
+					if (!ok) continue;
+					last_p0 = p0;
+					HIP_CHECK(hipEventSynchronize(e1));
+					float first_ms = 0.0f;
+					HIP_CHECK(hipEventElapsedTime(&first_ms, e0, e1));
+					if (bp0 < 0 && first_ms > 2.0f * kTuneLimitMs) {
+						best = first_ms;
+						bp0 = p0;
+						bp1 = built;
+						break;
+					}
 					for (int kb : kbs)
 						for (int tc : {8, 4}) {   // pairs per chunk (8 first: the better one when it fits)
 							if (bp0 >= 0 && best > kTuneLimitMs) break;
 							const int p1 = tc | (kb << 16);
 							HIP_CHECK(hipEventRecord(e0, s));
-							const bool ok = launch_tiled(g, p0, p1, scratch, a, d_delay, s);
+							const bool run = launch_tiled(g, p0, p1, scratch, a, d_delay, s);
 							HIP_CHECK(hipEventRecord(e1, s));
 							HIP_CHECK(hipEventSynchronize(e1));
 							float ms = 0.0f;
 							HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-							if (ok && (bp0 < 0 || ms < best)) {
+							if (run && (bp0 < 0 || ms < best)) {
 								best = ms;
 								bp0 = p0;
 								bp1 = p1;
@@ -380,8 +409,9 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 				scratch.tuned_key = key;
 				scratch.tuned_p0 = bp0;
 				scratch.tuned_p1 = bp1;
-				// the winner runs once more so that its tables stay resident
-				if (bp0 >= 0 && launch_tiled(g, bp0, bp1, scratch, a, d_delay, s)) return;
+				// every candidate wrote the same images; the winner runs once
+				// more only if another shape's tables replaced its own
+				if (bp0 >= 0 && (bp0 == last_p0 || launch_tiled(g, bp0, bp1, scratch, a, d_delay, s))) return;
 			} else if (scratch.tuned_p0 >= 0 &&
 					launch_tiled(g, scratch.tuned_p0, scratch.tuned_p1, scratch, a, d_delay, s)) {
 				return;
