@@ -90,11 +90,17 @@ struct mista_handle {
 		gexec = nullptr;
 	}
 
+	// host path: echoes uploaded chunk by chunk on their own stream
+	hipStream_t copy_stream = nullptr;
+	std::vector<hipEvent_t> chunk_ev;
+
 	~mista_handle()
 	{
 		drop_graph();
 		for (auto& e : ev)
 			if (e) (void)hipEventDestroy(e);
+		for (auto& e : chunk_ev) (void)hipEventDestroy(e);
+		if (copy_stream) (void)hipStreamDestroy(copy_stream);
 		if (stream) (void)hipStreamDestroy(stream);
 	}
 };
@@ -748,11 +754,39 @@ int mista_process(mista_handle* h, const float* echoes, uint32_t num_samples, fl
 		hipStream_t s = h->stream;
 		const uint64_t n = static_cast<uint64_t>(ntx_of(h->cfg)) * h->cfg.num_elements * num_samples;
 		float* d_raw = static_cast<float*>(h->d_raw.reserve(n * sizeof(float)));
-		HIP_CHECK(hipMemcpyAsync(d_raw, echoes, n * sizeof(float), hipMemcpyHostToDevice, s));
 		const uint64_t npix = static_cast<uint64_t>(h->n1) * h->n2;
 		float* d_value = static_cast<float*>(h->d_value.reserve(npix * sizeof(float)));
 		float* d_factor = static_cast<float*>(h->d_factor.reserve(npix * sizeof(float)));
-		run_prepare(h, d_raw, num_samples, s);
+		if (h->rf || num_samples == 0) {
+			HIP_CHECK(hipMemcpyAsync(d_raw, echoes, n * sizeof(float), hipMemcpyHostToDevice, s));
+			run_prepare(h, d_raw, num_samples, s);
+		} else {
+			// The echoes go up in the chunks the preparation consumes, on a
+			// copy stream; each chunk's kernels wait for that chunk alone, so
+			// the upload overlaps the preparation of the chunks before it.
+			if (!h->copy_stream) HIP_CHECK(hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+			size_t k = 0;
+			h->prep.before_chunk = [&](uint32_t c0, uint32_t nc) {
+				if (k == h->chunk_ev.size()) {
+					hipEvent_t e = nullptr;
+					HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+					h->chunk_ev.push_back(e);
+				}
+				const uint64_t off = static_cast<uint64_t>(c0) * num_samples;
+				HIP_CHECK(hipMemcpyAsync(d_raw + off, echoes + off, static_cast<uint64_t>(nc) * num_samples * sizeof(float),
+						hipMemcpyHostToDevice, h->copy_stream));
+				HIP_CHECK(hipEventRecord(h->chunk_ev[k], h->copy_stream));
+				HIP_CHECK(hipStreamWaitEvent(s, h->chunk_ev[k], 0));
+				++k;
+			};
+			try {
+				run_prepare(h, d_raw, num_samples, s);
+			} catch (...) {
+				h->prep.before_chunk = nullptr;
+				throw;
+			}
+			h->prep.before_chunk = nullptr;
+		}
 		run_das(h, d_value, d_factor, s);
 		HIP_CHECK(hipMemcpyAsync(value, d_value, npix * sizeof(float), hipMemcpyDeviceToHost, s));
 		if (factor) HIP_CHECK(hipMemcpyAsync(factor, d_factor, npix * sizeof(float), hipMemcpyDeviceToHost, s));

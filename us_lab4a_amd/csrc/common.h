@@ -7,6 +7,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -140,6 +141,9 @@ struct PrepState {
 	PrepPlan* plan = nullptr;  // owned; destroyed in prep.hip where PrepPlan is complete
 	int method = 0;            // 0 automatic, 1 library FFT (hipFFT), 2 fused LDS kernel
 	bool last_fused = false;   // the last preparation ran the fused kernel
+	// called before the kernels of channels [c0, c0 + n) are launched (the
+	// host path uploads each chunk of echoes just in time)
+	std::function<void(uint32_t c0, uint32_t n)> before_chunk;
 	PrepState() = default;
 	PrepState(const PrepState&) = delete;
 	PrepState& operator=(const PrepState&) = delete;

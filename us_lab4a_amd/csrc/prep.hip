@@ -880,6 +880,7 @@ static void prepare_fused(PrepState& st, const FrameGeom& g, const float* d_raw,
 	A.plan = P.fplan;
 	for (uint32_t c0 = 0; c0 < nch; c0 += chunk) {
 		const uint32_t n = std::min(chunk, nch - c0);
+		if (st.before_chunk) st.before_chunk(c0, n);
 		A.raw = d_raw + static_cast<uint64_t>(c0) * g.ns;
 		if (scaled) {
 			A.out = nullptr;
@@ -944,6 +945,7 @@ void prepare_signals(PrepState& st, const FrameGeom& g, const float* d_raw, cons
 
 	for (uint32_t c0 = 0; c0 < nch; c0 += chunk) {
 		const uint32_t n = std::min(chunk, nch - c0);
+		if (st.before_chunk) st.before_chunk(c0, n);
 		const float* raw = d_raw + static_cast<uint64_t>(c0) * g.ns;
 		if (g.up == 1) {
 			hipLaunchKernelGGL(copy_dc_kernel, dim3(n), dim3(kFirThreads), 0, s, raw, g.L, g.nf, g.dz_up, real);
