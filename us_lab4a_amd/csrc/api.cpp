@@ -561,11 +561,13 @@ int mista_set_rx_apod(mista_handle* h, const float* rx_apod, uint32_t n)
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
 	return guarded(h, [&] {
-		++h->settings_epoch;
-		++h->settings_epoch;
 		if (!rx_apod) MISTA_THROW(MISTA_E_INVALID_PARAM, "rx_apod is null.");
 		if (n != h->cfg.num_elements)
 			MISTA_THROW(MISTA_E_INVALID_VALUE, "Wrong rx apodization size: " << n << " (should be " << h->cfg.num_elements << ").");
+		// unchanged weights (a request loop sets them for every job): no
+		// upload, no stream synchronisation, no new frame graph
+		if (h->rx_apod.size() == n && std::equal(h->rx_apod.begin(), h->rx_apod.end(), rx_apod)) return;
+		++h->settings_epoch;
 		h->rx_apod.assign(rx_apod, rx_apod + n);
 		h->weights_dirty = true;
 	});
@@ -575,16 +577,19 @@ int mista_set_tx_apod(mista_handle* h, const float* tx_apod, uint32_t n)
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
 	return guarded(h, [&] {
-		++h->settings_epoch;
 		if (!tx_apod) {
+			if (!h->tx_apod_set) return;
 			h->tx_apod.clear();
 			h->tx_apod_set = false;
 		} else {
 			if (n != ntx_of(h->cfg))
 				MISTA_THROW(MISTA_E_INVALID_VALUE, "Wrong tx apodization size: " << n << " (should be " << ntx_of(h->cfg) << ").");
+			if (h->tx_apod_set && h->tx_apod.size() == n && std::equal(h->tx_apod.begin(), h->tx_apod.end(), tx_apod))
+				return;
 			h->tx_apod.assign(tx_apod, tx_apod + n);
 			h->tx_apod_set = true;
 		}
+		++h->settings_epoch;
 		h->weights_dirty = true;
 	});
 }
