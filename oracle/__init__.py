@@ -2,8 +2,9 @@
 
 The oracle is a C++ restatement of the reference CPU STA processor
 (src/user/processor/VectorialSTAProcessor.h:158-341) and its DSP helpers.
-Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may use
-it, as the checker or as the reported CPU baseline.  The product library
+Only tests/, the smoke check and bench.py's CPU-baseline leg may use it, as
+the reference the GPU results are compared with or as the reported CPU
+baseline.  The product library
 (us_lab4a_amd) never imports it.
 """
 import ctypes

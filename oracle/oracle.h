@@ -5,8 +5,9 @@
  * (src/user/processor/VectorialSTAProcessor.h:158-341) and of the DSP
  * routines it calls, in FP32 ("reference-equivalent", same loop order and
  * arithmetic form) and FP64 (accuracy oracle).  Only tests/, the smoke check
- * and bench.py's CPU-baseline leg may load it, and only as the checker or the
- * reported CPU baseline; the product path never calls it.
+ * and bench.py's CPU-baseline leg may load it, and only as the reference the
+ * GPU results are compared with or as the reported CPU baseline; the product
+ * path never calls it.
  *
  * Every function cites the reference file:line it follows.  Which checks pin
  * which function is listed in the header of oracle.cpp.
