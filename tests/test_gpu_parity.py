@@ -242,7 +242,7 @@ def test_graded_shape_columns_match_oracle():
 
 
 @pytest.mark.parametrize("tz,storage,chunking", [(0, "f32", 4), (2, "f32", 4), (3, "f32", 4), (0, "f16", 4),
-                                                 (0, "bf16", 4), (5, "f32", 8 | (150 << 16)),
+                                                 (0, "bf16", 4), (5, "bf16", 8 | (150 << 16)),
                                                  (6, "f16", 8 | (150 << 16))])
 def test_tiled_kernel_equals_gather_kernel(small, tz, storage, chunking):
     """The LDS-tiled kernel (256-lane tiles 16x16, 8x32, 32x8 and 512-lane
@@ -263,6 +263,7 @@ def test_tiled_kernel_equals_gather_kernel(small, tz, storage, chunking):
         p.das_device(dv, df)
         torch.cuda.synchronize()
         out[variant] = (dv.cpu().numpy(), df.cpu().numpy())
+        assert (p.last_kernel()[0] >= 0) == (variant == 2)   # the tiled kernel ran, not the fallback
     assert np.array_equal(out[1][0], out[2][0]) and np.array_equal(out[1][1], out[2][1])
     p.close()
     wl.cf = "none"
@@ -392,6 +393,7 @@ def test_tiled_kernel_over_128_transmits(cf):
         p.das_device(dv, df)
         torch.cuda.synchronize()
         out[variant] = (dv.cpu().numpy().reshape(wl.n_x, wl.n_z), df.cpu().numpy().reshape(wl.n_x, wl.n_z))
+        assert (p.last_kernel()[0] >= 0) == (variant == 2)   # the tiled kernel ran, not the fallback
     p.set_kernel(0)
     v_a, f_a = p.process_echoes(echoes)
     tuned = p.tuned_kernel()

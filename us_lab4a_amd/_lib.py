@@ -186,6 +186,7 @@ SIGNATURES = {
     "mista_query": (_i32, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u64)]),
     "mista_set_kernel": (_i32, [_vp, _i32, _i32, _i32]),
     "mista_tuned_kernel": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "mista_last_kernel": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "mista_set_graph": (_i32, [_vp, _i32]),
     "mista_graph_stats": (_i32, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
     "mista_last_timing": (_i32, [_vp, _fp, _fp]),

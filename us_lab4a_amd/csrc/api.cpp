@@ -904,6 +904,14 @@ int mista_tuned_kernel(const mista_handle* h, int* p0, int* p1)
 	return MISTA_OK;
 }
 
+int mista_last_kernel(const mista_handle* h, int* tile_param0, int* tile_param1)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	if (tile_param0) *tile_param0 = h->das_scratch.last_p0;
+	if (tile_param1) *tile_param1 = h->das_scratch.last_p1;
+	return MISTA_OK;
+}
+
 int mista_set_prep(mista_handle* h, int method)
 {
 	if (!h) return MISTA_E_INVALID_PARAM;

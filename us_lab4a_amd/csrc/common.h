@@ -183,6 +183,7 @@ struct DasScratch {
 	uint32_t max_slots = 0;
 	uint64_t tuned_key = ~0ull;     // geometry the autotuner last chose for
 	int tuned_p0 = -1, tuned_p1 = 0;
+	int last_p0 = -1, last_p1 = 0;  // the last DAS launch (-1: gather kernel)
 };
 // Bytes of zeroed padding after the analytic tensor (window over-read).
 size_t tensor_tail_padding();

@@ -293,6 +293,8 @@ bool launch_tiled(const FrameGeom& g, int p0, int p1, DasScratch& scratch, const
 	A.wcap = scratch.wcap;
 	A.ntx_pad = g.ntx_pad;
 	A.nbuf = nbuf;
+	scratch.last_p0 = p0 == 0 ? (ny > 1 ? 4 : 1) : p0;
+	scratch.last_p1 = static_cast<int>(tc | (nbuf << 8) | (want_kb << 16));
 	switch (g.storage) {
 	case MISTA_STORAGE_F32: launch_tile_storage<MISTA_STORAGE_F32>(g.cf_kind, A, ntiles, tc, s); break;
 	case MISTA_STORAGE_F16: launch_tile_storage<MISTA_STORAGE_F16>(g.cf_kind, A, ntiles, tc, s); break;
@@ -418,6 +420,8 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 			}
 		}
 	}
+	scratch.last_p0 = -1;
+	scratch.last_p1 = 0;
 	switch (g.storage) {
 	case MISTA_STORAGE_F32: launch_gather<MISTA_STORAGE_F32>(g, a, s); break;
 	case MISTA_STORAGE_F16: launch_gather<MISTA_STORAGE_F16>(g, a, s); break;

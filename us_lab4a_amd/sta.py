@@ -168,6 +168,12 @@ class HipSTAProcessor:
         check(lib().mista_graph_stats(self._h, ctypes.byref(a), ctypes.byref(b)), self._h)
         return a.value, b.value
 
+    def last_kernel(self):
+        """(tile shape, chunking) of the last DAS launch; shape -1: gather."""
+        a, b = ctypes.c_int32(0), ctypes.c_int32(0)
+        check(lib().mista_last_kernel(self._h, ctypes.byref(a), ctypes.byref(b)), self._h)
+        return a.value, b.value
+
     def tuned_kernel(self):
         """(tile shape, chunking) the automatic variant chose; (-1, 0) when
         nothing was tuned or the gather kernel runs."""
