@@ -177,8 +177,9 @@ __global__ void __launch_bounds__(256) window_params_kernel(const float2* __rest
 			const float lo = (offset + bt.x) + br.x;
 			const float hi = (offset + bt.y) + br.y;
 			const uint32_t chan = t * nrx + r;
-			// lo past the record end, negative or NaN: nothing staged, every
-			// lane of the pair is excluded by the predicated path
+			// lo past the record end or NaN: nothing staged, every lane of
+			// the pair is excluded by the predicated path.  (lo is never
+			// negative: delays are >= 0 and the peak offset is >= 0.)
 			if (lo >= 0.0f && lo < static_cast<float>(L)) {
 				const uint32_t ws = static_cast<uint32_t>(floorf(lo));
 				const float fhi = floorf(hi);

@@ -28,13 +28,34 @@ __device__ __forceinline__ float2 load_sample(const void* __restrict__ t, uint64
 
 // LDS entry type per storage kind and its conversion to FP32 complex.
 template<int STORAGE> struct Entry;
+// get: one entry as FP32 (re, im).  lerp: v0 + k (v1 - v0) per component
+// for the entries e[0], e[1], with FP32 arithmetic: the difference rounded
+// once, then one fused multiply-add.
 template<> struct Entry<MISTA_STORAGE_F32> {
 	using T = float2;
 	static __device__ __forceinline__ float2 get(T v) { return v; }
+	static __device__ __forceinline__ float2 lerp(const T* e, float k)
+	{
+		const float2 v0 = e[0], v1 = e[1];
+		return make_float2(fmaf(k, v1.x - v0.x, v0.x), fmaf(k, v1.y - v0.y, v0.y));
+	}
 };
 template<> struct Entry<MISTA_STORAGE_F16> {
 	using T = __half2;
 	static __device__ __forceinline__ float2 get(T v) { return __half22float2(v); }
+	static __device__ __forceinline__ float2 lerp(const T* e, float k)
+	{
+		const T h0 = e[0], h1 = e[1];
+		const uint32_t u0 = __builtin_bit_cast(uint32_t, h0), u1 = __builtin_bit_cast(uint32_t, h1);
+		// v1 - v0 straight from the FP16 halves: the mixed-precision FMA
+		// widens both exactly and rounds 1 * v1 - v0 once to FP32, as the
+		// widen-then-subtract form does, in one instruction
+		float dx, dy;
+		asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,1]" : "=v"(dx) : "v"(u1), "v"(u0));
+		asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[1,0,1] op_sel_hi:[1,0,1]" : "=v"(dy) : "v"(u1), "v"(u0));
+		const float2 v0 = __half22float2(h0);
+		return make_float2(fmaf(k, dx, v0.x), fmaf(k, dy, v0.y));
+	}
 };
 template<> struct Entry<MISTA_STORAGE_BF16> {
 	using T = uint32_t;
@@ -42,11 +63,21 @@ template<> struct Entry<MISTA_STORAGE_BF16> {
 	{
 		return make_float2(__uint_as_float(b << 16), __uint_as_float(b & 0xFFFF0000u));
 	}
+	static __device__ __forceinline__ float2 lerp(const T* e, float k)
+	{
+		const float2 v0 = get(e[0]), v1 = get(e[1]);
+		return make_float2(fmaf(k, v1.x - v0.x, v0.x), fmaf(k, v1.y - v0.y, v0.y));
+	}
 };
 
 template<> struct Entry<kStorageReal> {
 	using T = float;
 	static __device__ __forceinline__ float2 get(T v) { return make_float2(v, 0.0f); }
+	static __device__ __forceinline__ float2 lerp(const T* e, float k)
+	{
+		const float v0 = e[0], v1 = e[1];
+		return make_float2(fmaf(k, v1 - v0, v0), 0.0f);
+	}
 };
 
 struct DasArgs {
@@ -349,27 +380,33 @@ das_tile_kernel(const TileArgs A)
 #pragma unroll
 				for (int j = 0; j < TC; ++j) {
 					// One form for every pair: lanes outside 0 <= pos < L-1 get
-					// weight 0; the entry index is clamped into the window,
-					// which changes nothing for in-range lanes (floor(pos) - ws
-					// lies in [0, need - 2] by the window construction) and
-					// keeps excluded lanes' reads inside the buffer.
-					const int ws = static_cast<int>(pr[j].word & kWsMask);
+					// weight 0; the entry index floor(pos) - ws is clamped,
+					// as an unsigned value, to the window, which changes
+					// nothing for in-range lanes (it lies in [0, need - 2] by
+					// the window construction) and keeps excluded lanes'
+					// reads on staged (finite) entries, whose products with
+					// weight 0 leave the sums as they are (a sum is never -0).
+					const uint32_t ws = pr[j].word & kWsMask;
 					const float pos = dtx[c * TC + j] + drx;
 					const float k = __builtin_amdgcn_fractf(pos);
-					const bool ok = pos >= 0.0f && pos < lim;
+					// 0 <= pos < lim as one unsigned compare of the bit
+					// patterns: positions are never -0 (delays are >= +0), so
+					// non-negative ones order like their bits, and negative
+					// values and NaN compare above lim's
+					const bool ok = __float_as_uint(pos) < __float_as_uint(lim);
 					// load the weight unconditionally (a select, not a branch
 					// around a scalar load)
 					float wv = wr[j];
 					asm volatile("" : "+s"(wv));
 					const float w = ok ? wv : 0.0f;
 					const float pc = __builtin_amdgcn_fmed3f(pos, 0.0f, lim);   // NaN-free, no int overflow
-					const int rel = min(max(static_cast<int>(pc) - ws, 0), static_cast<int>(wcap) - 2);
-					const int off = rel * static_cast<int>(ESZ) + static_cast<int>(bbase + j * wcap * ESZ);
-					const E* e = reinterpret_cast<const E*>(smem + off);
-					const float2 v0 = Entry<STORAGE>::get(e[0]);
-					const float2 v1 = Entry<STORAGE>::get(e[1]);
-					ar = fmaf(fmaf(k, v1.x - v0.x, v0.x), w, ar);
-					ai = fmaf(fmaf(k, v1.y - v0.y, v0.y), w, ai);
+					const uint32_t rel = min(static_cast<uint32_t>(static_cast<int>(pc)) - ws, wcap - 2u);
+					uint32_t sb = bbase + j * wcap * ESZ;   // uniform: one shift-add per lane
+					asm("" : "+s"(sb));
+					const E* e = reinterpret_cast<const E*>(smem + (rel * ESZ + sb));
+					const float2 v = Entry<STORAGE>::lerp(e, k);
+					ar = fmaf(v.x, w, ar);
+					ai = fmaf(v.y, w, ai);
 				}
 				asm volatile("" : "+v"(ar), "+v"(ai));
 				// after this chunk's LDS reads (a pending scalar load would
