@@ -267,12 +267,14 @@ __device__ inline uint32_t pack_f16x2(float re, float im)
 constexpr int kFftThreads = 1024;
 constexpr uint32_t kFirJ = 27;   // taps per phase of the Kaiser upsampling design (every U)
 constexpr uint32_t kFftMaxM = 18432;
+static_assert(kFftMaxM < (1u << 15), "the FFT passes divide by multiply-high, exact below 2^15");
 constexpr int kFftMaxPasses = 16;
 
 struct FftPlanArgs {
 	uint32_t M, npass;
 	uint32_t radix[kFftMaxPasses];
 	uint32_t ns[kFftMaxPasses];        // sub-transform size before the pass
+	uint32_t ns_magic[kFftMaxPasses];  // ceil(2^32 / ns) (0 when ns = 1): b / ns = mulhi(b, ns_magic)
 	uint32_t tw_off[kFftMaxPasses];    // offset of the pass's twiddles in tw
 };
 
@@ -365,7 +367,8 @@ template<> __device__ __forceinline__ void dft<5>(float2* v)
 }
 
 template<int R>
-__device__ __forceinline__ void fft_pass(float2* D, uint32_t M, uint32_t Ns, const float2* __restrict__ tw)
+__device__ __forceinline__ void fft_pass(float2* D, uint32_t M, uint32_t Ns, uint32_t magic,
+		const float2* __restrict__ tw)
 {
 	constexpr int MAXB = static_cast<int>((kFftMaxM / R + kFftThreads - 1) / kFftThreads);
 	const uint32_t nb = M / R;
@@ -383,12 +386,16 @@ __device__ __forceinline__ void fft_pass(float2* D, uint32_t M, uint32_t Ns, con
 	for (int t = 0; t < MAXB; ++t) {
 		const uint32_t b = threadIdx.x + static_cast<uint32_t>(t) * kFftThreads;
 		if (b < nb) {
-			const uint32_t j = b % Ns;
+			// q = b / Ns by a multiply-high: with magic = ceil(2^32 / Ns) =
+			// (2^32 + e) / Ns, b * magic / 2^32 = b / Ns + b e / (Ns 2^32),
+			// and b e < 2^32 (b, e < 2^15) keeps the floor exact
+			const uint32_t q = (Ns == 1) ? b : __umulhi(b, magic);
+			const uint32_t j = b - q * Ns;
 			const float2* w = tw + j * (R - 1);
 #pragma unroll
 			for (int i = 1; i < R; ++i) v[t][i] = cmul(v[t][i], w[i - 1]);
 			dft<R>(v[t]);
-			const uint32_t o = (b / Ns) * Ns * R + j;
+			const uint32_t o = q * Ns * R + j;
 #pragma unroll
 			for (int i = 0; i < R; ++i) D[o + i * Ns] = v[t][i];
 		}
@@ -401,11 +408,11 @@ __device__ void fft_forward(float2* D, const FftPlanArgs& P, const float2* __res
 	for (uint32_t p = 0; p < P.npass; ++p) {
 		const float2* t = tw + P.tw_off[p];
 		switch (P.radix[p]) {
-		case 8: fft_pass<8>(D, P.M, P.ns[p], t); break;
-		case 4: fft_pass<4>(D, P.M, P.ns[p], t); break;
-		case 2: fft_pass<2>(D, P.M, P.ns[p], t); break;
-		case 3: fft_pass<3>(D, P.M, P.ns[p], t); break;
-		default: fft_pass<5>(D, P.M, P.ns[p], t); break;
+		case 8: fft_pass<8>(D, P.M, P.ns[p], P.ns_magic[p], t); break;
+		case 4: fft_pass<4>(D, P.M, P.ns[p], P.ns_magic[p], t); break;
+		case 2: fft_pass<2>(D, P.M, P.ns[p], P.ns_magic[p], t); break;
+		case 3: fft_pass<3>(D, P.M, P.ns[p], P.ns_magic[p], t); break;
+		default: fft_pass<5>(D, P.M, P.ns[p], P.ns_magic[p], t); break;
 		}
 	}
 }
@@ -826,6 +833,8 @@ static void build_fused_plan(PrepPlan& P, uint32_t nf, hipStream_t s)
 		const uint32_t R = rad[p];
 		F.radix[p] = R;
 		F.ns[p] = Ns;
+		// exact division by multiply-high needs b, Ns < 2^15 (b < M / R)
+		F.ns_magic[p] = Ns == 1 ? 0u : static_cast<uint32_t>(((1ull << 32) + Ns - 1) / Ns);
 		F.tw_off[p] = static_cast<uint32_t>(tw.size());
 		for (uint32_t j = 0; j < Ns; ++j)
 			for (uint32_t i = 1; i < R; ++i) {
