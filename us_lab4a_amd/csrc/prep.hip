@@ -599,12 +599,44 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 		}
 		return;
 	}
-	for (uint32_t m = tid; 2 * m < L; m += kFftThreads) put(2 * m, D[m]);
-	__syncthreads();
-	for (uint32_t m = tid; m < M; m += kFftThreads) D[m] = bo[m];
+	if (mag) {
+		for (uint32_t m = tid; 2 * m < L; m += kFftThreads) put(2 * m, D[m]);
+		__syncthreads();
+		for (uint32_t m = tid; m < M; m += kFftThreads) D[m] = bo[m];
+		__syncthreads();
+		fft_forward(D, A.plan, A.tw);
+		for (uint32_t m = tid; 2 * m + 1 < L; m += kFftThreads) put(2 * m + 1, D[m]);
+		return;
+	}
+	// F32 complex: the even samples wait in the scratch row while the odd
+	// transform runs, swapped in per index with the odd spectrum (index m is
+	// read and written by one thread only), so that every output pair
+	// (z[2m], z[2m+1]) is then written at once, 16 contiguous bytes per lane,
+	// instead of two half-filled passes over the row.
+	for (uint32_t m = tid; m < M; m += kFftThreads) {
+		const float2 od = bo[m];
+		if (2 * m < L) bo[m] = D[m];
+		D[m] = od;
+	}
 	__syncthreads();
 	fft_forward(D, A.plan, A.tw);
-	for (uint32_t m = tid; 2 * m + 1 < L; m += kFftThreads) put(2 * m + 1, D[m]);
+	const bool pairs16 = (L & 1u) == 0;   // rows start 16-byte aligned
+	for (uint32_t m = tid; 2 * m < L; m += kFftThreads) {
+		const float2 e = bo[m];
+		const float2 ev = make_float2(e.x * coef, -e.y * coef);
+		if (2 * m + 1 < L) {
+			const float2 o = D[m];
+			const float2 ov = make_float2(o.x * coef, -o.y * coef);
+			if (pairs16) {
+				reinterpret_cast<float4*>(dst)[m] = make_float4(ev.x, ev.y, ov.x, ov.y);
+			} else {
+				dst[2 * m] = ev;
+				dst[2 * m + 1] = ov;
+			}
+		} else {
+			dst[2 * m] = ev;
+		}
+	}
 }
 
 // ---------------------------------------------------------------- storage
