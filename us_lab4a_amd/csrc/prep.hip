@@ -550,26 +550,20 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 	if (A.out16) {
 		// FP16/BF16 storage over the channel scale s = max(|re|, |im|) (1 if
 		// zero), stored value z/s.  The even samples wait, in FP32, in the
-		// words of the channel's own output row that their pair will occupy
-		// (word 2m, 2m+1 <- z[2m]), while the odd transform runs; each thread
-		// later reads back only what it wrote.  An odd L's last even sample
-		// stays in a register.
-		float* park = reinterpret_cast<float*>(A.out16 + static_cast<uint64_t>(ch) * L);
+		// scratch row, swapped per index with the odd spectrum as it comes
+		// into LDS (index m is read and written by one thread only), while
+		// the odd transform runs; then each output pair is written at once.
 		float mx = 0.0f;
-		float2 last = make_float2(0.0f, 0.0f);
-		for (uint32_t m = tid; 2 * m < L; m += kFftThreads) {
-			const float2 v = D[m];
-			const float re = v.x * coef, im = -v.y * coef;
-			mx = fmaxf(mx, fmaxf(fabsf(re), fabsf(im)));
-			if (2 * m + 1 < L) {
-				park[2 * m] = re;
-				park[2 * m + 1] = im;
-			} else {
-				last = make_float2(re, im);
+		for (uint32_t m = tid; m < M; m += kFftThreads) {
+			const float2 od = bo[m];
+			if (2 * m < L) {
+				const float2 v = D[m];
+				const float2 e = make_float2(v.x * coef, -v.y * coef);
+				mx = fmaxf(mx, fmaxf(fabsf(e.x), fabsf(e.y)));
+				bo[m] = e;
 			}
+			D[m] = od;
 		}
-		__syncthreads();
-		for (uint32_t m = tid; m < M; m += kFftThreads) D[m] = bo[m];
 		__syncthreads();
 		fft_forward(D, A.plan, A.tw);
 		for (uint32_t m = tid; 2 * m + 1 < L; m += kFftThreads) {
@@ -586,15 +580,22 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 		if (tid == 0) A.scale[ch] = sc;
 		const bool bf = (A.flags & kOutBf16) != 0;
 		uint32_t* row = A.out16 + static_cast<uint64_t>(ch) * L;
+		const bool pairs8 = (L & 1u) == 0;   // rows start 8-byte aligned
 		for (uint32_t m = tid; 2 * m < L; m += kFftThreads) {
+			const float2 e = bo[m];
+			const uint32_t pe = bf ? pack_bf16x2(e.x * inv, e.y * inv) : pack_f16x2(e.x * inv, e.y * inv);
 			if (2 * m + 1 < L) {
-				const float er = park[2 * m], ei = park[2 * m + 1];
 				const float2 v = D[m];
 				const float orr = v.x * coef, oi = -v.y * coef;
-				row[2 * m] = bf ? pack_bf16x2(er * inv, ei * inv) : pack_f16x2(er * inv, ei * inv);
-				row[2 * m + 1] = bf ? pack_bf16x2(orr * inv, oi * inv) : pack_f16x2(orr * inv, oi * inv);
+				const uint32_t po = bf ? pack_bf16x2(orr * inv, oi * inv) : pack_f16x2(orr * inv, oi * inv);
+				if (pairs8) {
+					reinterpret_cast<uint2*>(row)[m] = make_uint2(pe, po);
+				} else {
+					row[2 * m] = pe;
+					row[2 * m + 1] = po;
+				}
 			} else {
-				row[2 * m] = bf ? pack_bf16x2(last.x * inv, last.y * inv) : pack_f16x2(last.x * inv, last.y * inv);
+				row[2 * m] = pe;
 			}
 		}
 		return;
