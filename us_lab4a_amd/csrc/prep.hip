@@ -455,6 +455,29 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 #pragma unroll
 			for (int j = 0; j < kFirJ; ++j) xw[j] = xrow[-j];
 			const int ibase = static_cast<int>(q * up) - static_cast<int>(A.off);
+			// U = 8 or 4 with the lane's outputs inside [dz, L) and 16-byte
+			// aligned: the U phases leave as float4 stores (one lane's
+			// outputs are contiguous; phase by phase the lanes would write
+			// 4U bytes apart, onto a few LDS banks)
+			if ((up == 8 || up == 4) && ibase >= static_cast<int>(A.dz) && ibase + static_cast<int>(up) <= static_cast<int>(L) &&
+					(ibase & 3) == 0) {
+#pragma unroll
+				for (uint32_t h = 0; h < 8; h += 4) {
+					if (h >= up) break;
+					float o[4];
+#pragma unroll
+					for (int k = 0; k < 4; ++k) {
+						const float* hrow = hg + (h + k) * kFirJ;
+						float acc = 0.0f;
+#pragma unroll
+						for (int j = 0; j < kFirJ; ++j) acc = fmaf(hrow[j], xw[j], acc);
+						o[k] = acc;
+						part += static_cast<double>(acc);
+					}
+					*reinterpret_cast<float4*>(Df + ibase + h) = make_float4(o[0], o[1], o[2], o[3]);
+				}
+				continue;
+			}
 			for (uint32_t ph = 0; ph < up; ++ph) {
 				const float* hrow = hg + ph * kFirJ;
 				float acc = 0.0f;
