@@ -88,7 +88,7 @@ def cpu_baseline(wl, threads):
     wl.echoes_gpu(d)
     echoes = d[:ntx_s].cpu().numpy()
     del d
-    cfg = wl.oracle_cfg()
+    cfg = oracle.cfg_for(wl)
     cfg.last_tx_elem = cfg.first_tx_elem + ntx_s - 1
     t0 = time.perf_counter()
     a = oracle.prepare(cfg, echoes, np.float32, nthreads=threads)
@@ -302,6 +302,15 @@ def main():
         prep_ms.append(p_ms)
         das_ms.append(k_ms)
     lat = np.array(lat)
+    if args.graph:
+        # replayed frame graphs have no stage split: take it from eager frames
+        proc.set_graph(False)
+        prep_ms, das_ms = [], []
+        for _ in range(max(3, min(args.steps, 10))):
+            proc.process_device(d_echoes, wl.num_samples, d_value, d_factor)
+            p_ms, k_ms = proc.last_timing()
+            prep_ms.append(p_ms)
+            das_ms.append(k_ms)
     # host-buffer frames (mista_process): echoes in from pinned host memory,
     # images back; reported beside the device-resident rate, never as value
     h_echoes = torch.empty(d_echoes.shape, dtype=torch.float32, pin_memory=True)

@@ -194,6 +194,11 @@ int mista_download_analytic(mista_handle* h, float* analytic_ri);
 /* Test access to the FP32 delay table [num_elements][n1*n2] (samples). */
 int mista_download_delays(mista_handle* h, float* delays);
 
+/* Test access to the delay-table columns of n grid points: delays[row][i]
+ * = table[row][points[i]] for every row (2-D: the elements; 3-D: the active
+ * rx, then the active tx). */
+int mista_download_delays_at(mista_handle* h, const uint64_t* points, uint64_t n, float* delays);
+
 /* Sizes of the current frame state: signal length L, FFT size, grid points. */
 int mista_query(const mista_handle* h, uint32_t* signal_length, uint32_t* fft_size, uint64_t* num_points);
 
@@ -207,9 +212,9 @@ int mista_set_kernel(mista_handle* h, int variant, int tile_param0, int tile_par
  * pointers and sample count, no new allocation, table, tuning or setting) is
  * recorded once as a hipGraph on the handle's stream, and later frames with
  * the same key replay it with one hipGraphLaunch.  Only the fused
- * preparation path is captured.  In graph frames mista_last_timing reports
- * the whole frame as DAS time.  mista_graph_stats counts captures and
- * replays. */
+ * preparation path is captured.  After a replayed frame mista_last_timing
+ * reports both stage times as -1 (not observable inside one graph launch).
+ * mista_graph_stats counts captures and replays. */
 int mista_set_graph(mista_handle* h, int enable);
 int mista_graph_stats(const mista_handle* h, uint64_t* captures, uint64_t* launches);
 
@@ -251,8 +256,9 @@ int mista_envelope(mista_handle* h, float* data, uint32_t n1, uint32_t n2);
  * 2 = fused LDS kernel (error if the frame does not fit). */
 int mista_set_prep(mista_handle* h, int method);
 
-/* Time of the last mista_das_device/mista_process_device DAS launch (ms,
- * measured with HIP events on the launch stream); 0 if not available. */
+/* Stage times of the last mista_das_device/mista_process_device frame (ms,
+ * measured with HIP events on the launch stream): preparation and DAS; 0
+ * when a stage did not run, -1 for both after a replayed frame graph. */
 int mista_last_timing(const mista_handle* h, float* prep_ms, float* das_ms);
 
 /* ---- serving engine (no reference counterpart) ----

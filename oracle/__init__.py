@@ -147,6 +147,11 @@ def lib():
         L.oracle_envelope_dim2_f32.argtypes = [_f32p, u32, u32, i32]
         L.oracle_envelope_dim2_f64.restype = i32
         L.oracle_envelope_dim2_f64.argtypes = [_f64p, u32, u32, i32]
+        L.oracle_delays_f32.restype = i32
+        L.oracle_delays_f32.argtypes = [P2, _f32p, u32, u32, _f32p]
+        L.oracle_das_table_f32.restype = i32
+        L.oracle_das_table_f32.argtypes = [P2, _f32p, u32, _f32p, u32, u32, ctypes.c_uint64, _f32p, opt32, _f32p,
+                                           _f32p, opt32, i32, i32]
         _lib = L
     return _lib
 
@@ -339,6 +344,68 @@ def das_rxsums(cfg, analytic_ri, grid_xz, rx_apod, tx_apod=None, nthreads=0):
     _check(lib().oracle_das_rxsums_f32(ctypes.byref(cfg), analytic_ri, L, grid_xz, n1, n2, rx_apod,
                                        opt[0] if opt else None, value, factor, sums, nthreads))
     return value, factor, sums[..., 0] + 1j * sums[..., 1]
+
+
+def delays(cfg, grid_xz):
+    """The FP32 delay table [num_elements][n1*n2] (samples) that `das`
+    computes per pixel: distance2DY0 * invCT in float (Geometry.h:55-65)."""
+    grid_xz = np.ascontiguousarray(grid_xz, np.float32)
+    n1, n2 = grid_xz.shape[:2]
+    out = np.empty((cfg.num_elements, n1 * n2), np.float32)
+    _check(lib().oracle_delays_f32(ctypes.byref(cfg), grid_xz, n1, n2, out))
+    return out
+
+
+FORM_CPU, FORM_CUDA = 0, 1
+
+
+def das_table(cfg, analytic_ri, delay_table, rx_apod, tx_apod=None, form=FORM_CUDA, rxsums=False, tx_row0=None,
+              nthreads=0):
+    """FP32 DAS over a given delay table [rows][npix] (tx t at row
+    tx_row0 + t, default first_tx_elem + t; rx r at row r).  form FORM_CPU keeps the CPU
+    processor's arithmetic (VectorialSTAProcessor.h:269-284), FORM_CUDA the
+    reference CUDA kernels' fused form (VectorialSTACUDAProcessor.cu:93-103,
+    :163-169).  Returns (value[npix], factor[npix]) and, with rxsums, the
+    per-rx complex sums [npix][num_elements]."""
+    analytic_ri = np.ascontiguousarray(analytic_ri, np.float32)
+    L = analytic_ri.shape[2]
+    d = np.ascontiguousarray(delay_table, np.float32)
+    rows, npix = d.shape
+    rx_apod = np.ascontiguousarray(rx_apod, np.float32)
+    opt = _opt(tx_apod)
+    value = np.empty(npix, np.float32)
+    factor = np.empty(npix, np.float32)
+    sums = np.empty((npix, cfg.num_elements, 2), np.float32) if rxsums else None
+    row0 = cfg.first_tx_elem if tx_row0 is None else tx_row0
+    _check(lib().oracle_das_table_f32(ctypes.byref(cfg), analytic_ri, L, d, rows, row0, npix, rx_apod,
+                                      opt[0] if opt else None, value, factor,
+                                      sums.ctypes.data_as(ctypes.c_void_p) if rxsums else None, form, nthreads))
+    if rxsums:
+        return value, factor, sums[..., 0] + 1j * sums[..., 1]
+    return value, factor
+
+
+def cfg_for(wl):
+    """Oracle parameters of a 2-D workload (us_lab4a_amd.workloads.Workload2D)."""
+    cf = {"none": CF_NONE, "phase_coherence_factor": CF_PCF, "sign_coherence_factor": CF_SCF}[wl.cf]
+    return make_cfg(wl.num_elements, wl.num_samples, wl.upsampling, wl.pitch, wl.fs, wl.c, wl.fc,
+                    first_tx=wl.first_tx, last_tx=(wl.num_elements - 1 if wl.last_tx < 0 else wl.last_tx),
+                    dead_zone_m=wl.dead_zone_m, peak_offset=wl.peak_offset, cf_kind=cf, cf_param=wl.cf_param)
+
+
+def cfg2d_of_3d(cfg3):
+    """2-D parameters for das_table over a 3-D handle's delay table (rows:
+    the active rx, then the active tx; call with tx_row0 = num_rx)."""
+    return Sta2d(cfg3.num_rx, 0, cfg3.num_tx - 1, cfg3.num_samples, cfg3.upsampling_factor, cfg3.cf_kind, 0.0,
+                 cfg3.sampling_frequency, cfg3.propagation_speed, cfg3.center_frequency, cfg3.dead_zone_m,
+                 cfg3.peak_offset, cfg3.cf_param)
+
+
+def cfg3d_for(wl):
+    """Oracle parameters of a 3-D workload (us_lab4a_amd.workloads.Workload3D)."""
+    cf = {"none": CF_NONE, "phase_coherence_factor": CF_PCF, "sign_coherence_factor": CF_SCF}[wl.cf]
+    return make_cfg3d(wl.num_tx, wl.num_rx, wl.num_samples, wl.upsampling, wl.fs, wl.c, wl.fc,
+                      dead_zone_m=wl.dead_zone_m, peak_offset=wl.peak_offset, cf_kind=cf, cf_param=wl.cf_param)
 
 
 def branch_sensitive(rx_sums, kind, rel=1e-2):

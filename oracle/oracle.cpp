@@ -647,6 +647,105 @@ void das(const oracle_sta2d& cfg, const T* analytic_ri, uint32_t L,
 	for (size_t i = 0; i < npix; ++i) value[i] *= scale;
 }
 
+/* Delay-and-sum over a given FP32 delay table delays[row][npix] in samples
+ * (tx t at row tx_row0 + t, rx r at row r; 2-D tables have tx_row0 =
+ * first_tx, 3-D tables of active elements list the rx rows first,
+ * Vectorial3DSTAProcessor.h:193-250), as the reference GPU processors
+ * take theirs from calculateDelaysSTAKernel (VectorialSTACUDAProcessor.cu:
+ * 30-53).  position = (offset + d_tx) + d_rx, skipped unless position >= 0
+ * and idx = trunc(position) <= L - 2 (.cu:86-90, :156-160).
+ * form 0: the CPU processor's arithmetic, ((1 - k) s[i] + k s[i+1]) * apod
+ *         then +=, unfused (VectorialSTAProcessor.h:269-284, :312-325).
+ * form 1: the CUDA kernels' arithmetic, v = v0 + k (v1 - v0) and
+ *         sum += v * apod (.cu:93-103, :163-169), which nvcc contracts to
+ *         fused multiply-adds (its default -fmad=true): fma(k, v1 - v0, v0),
+ *         then fma(v, apod, sum).
+ * With a coherence factor the per-rx sums accumulate over tx in ascending
+ * order, then the value is |sum over rx| (.cu:172-178).  apod = rx apod,
+ * times the optional tx apod (an extension) as one float product. */
+template<typename T>
+void das_table(const oracle_sta2d& cfg, const T* analytic_ri, uint32_t L, const float* delays, uint32_t rows,
+		uint32_t tx_row0, uint64_t npix, const float* rx_apod_f, const float* tx_apod_f, T* value, T* factor, int form,
+		int nthreads, T* rxsum_out = nullptr)
+{
+	using C = std::complex<T>;
+	const C* tensor = reinterpret_cast<const C*>(analytic_ri);
+	const unsigned N = cfg.num_elements;
+	const unsigned first = cfg.first_tx_elem, last = cfg.last_tx_elem;
+	const unsigned ntx = last - first + 1U;
+	if (rows < N || rows < tx_row0 + ntx) throw std::invalid_argument("delay table has too few rows");
+	std::vector<T> w(static_cast<size_t>(ntx) * N);
+	for (unsigned t = 0; t < ntx; ++t)
+		for (unsigned r = 0; r < N; ++r)
+			w[static_cast<size_t>(t) * N + r] = tx_apod_f ? static_cast<T>(rx_apod_f[r] * tx_apod_f[t])
+					: static_cast<T>(rx_apod_f[r]);
+	const T fs = static_cast<T>(cfg.sampling_frequency);
+	const T fc = static_cast<T>(cfg.center_frequency);
+	const T offset = (fs * cfg.upsampling_factor) * static_cast<T>(cfg.peak_offset) / fc; /* :147 */
+	const int cf = cfg.cf_kind;
+	const T cf_param = static_cast<T>(cfg.cf_param);
+	const int nt = num_threads_or_default<T>(nthreads);
+	auto sample = [&](const C* p, T k) -> C {
+		if (form == 1) {
+			const C v0 = p[0], v1 = p[1];
+			return C(std::fma(k, v1.real() - v0.real(), v0.real()), std::fma(k, v1.imag() - v0.imag(), v0.imag()));
+		}
+		return (1 - k) * p[0] + k * p[1];
+	};
+	auto accumulate = [&](C& acc, C v, T wv) {
+		if (form == 1) acc = C(std::fma(v.real(), wv, acc.real()), std::fma(v.imag(), wv, acc.imag()));
+		else acc += v * wv;
+	};
+#pragma omp parallel num_threads(nt)
+	{
+		std::vector<C> rx_sum(N);
+		std::vector<T> re(N);
+		Pcf<T> pcf(cf_param);
+#pragma omp for schedule(dynamic, 64)
+		for (long long q = 0; q < static_cast<long long>(npix); ++q) {
+			const size_t pt = static_cast<size_t>(q);
+			auto term = [&](unsigned ltx, unsigned rx, C& acc) {
+				const T d = (offset + static_cast<T>(delays[static_cast<size_t>(tx_row0 + ltx) * npix + pt])) +
+						static_cast<T>(delays[static_cast<size_t>(rx) * npix + pt]);
+				if (!(d >= 0)) return;
+				const size_t idx = static_cast<size_t>(d);
+				if (idx + 1U >= L) return;
+				const T k = d - static_cast<T>(idx);
+				accumulate(acc, sample(tensor + (static_cast<size_t>(ltx) * N + rx) * L + idx, k),
+						w[static_cast<size_t>(ltx) * N + rx]);
+			};
+			if (cf != ORACLE_CF_NONE) {
+				std::fill(rx_sum.begin(), rx_sum.end(), C(0));
+				for (unsigned ltx = 0; ltx < ntx; ++ltx)
+					for (unsigned rx = 0; rx < N; ++rx) term(ltx, rx, rx_sum[rx]);
+				if (rxsum_out) {
+					for (unsigned rx = 0; rx < N; ++rx) {
+						rxsum_out[(pt * N + rx) * 2] = rx_sum[rx].real();
+						rxsum_out[(pt * N + rx) * 2 + 1] = rx_sum[rx].imag();
+					}
+				}
+				if (cf == ORACLE_CF_PCF) {
+					factor[pt] = pcf(rx_sum.data(), N);
+				} else {
+					for (unsigned rx = 0; rx < N; ++rx) re[rx] = rx_sum[rx].real();
+					factor[pt] = scf<T>(re.data(), N, cf_param);
+				}
+				C s(0);
+				for (unsigned rx = 0; rx < N; ++rx) s += rx_sum[rx];
+				value[pt] = std::abs(s);
+			} else {
+				C s(0);
+				for (unsigned ltx = 0; ltx < ntx; ++ltx)
+					for (unsigned rx = 0; rx < N; ++rx) term(ltx, rx, s);
+				value[pt] = std::abs(s);
+				factor[pt] = 1;
+			}
+		}
+	}
+	const T scale = T(1) / (static_cast<size_t>(ntx) * N); /* :337-338 */
+	for (uint64_t i = 0; i < npix; ++i) value[i] *= scale;
+}
+
 /* Vectorial3DSTAProcessor.h:193-250: separate tx/rx element lists,
  * Geometry::distance3DZ0 (Geometry.h:77-84). */
 template<typename T>
@@ -932,6 +1031,49 @@ int oracle_das_rxsums_f32(const oracle_sta2d* cfg, const float* analytic, uint32
 	}
 	return guarded([&] {
 		das<float>(*cfg, analytic, L, grid_xz, n1, n2, rx_apod, tx_apod, value, factor, nthreads, rxsums);
+	});
+}
+
+int oracle_delays_f32(const oracle_sta2d* cfg, const float* grid_xz, uint32_t n1, uint32_t n2, float* delays)
+{
+	if (check_cfg(cfg)) return -1;
+	return guarded([&] {
+		/* the per-pixel delays of das<float>: ArrayGeometry.h:37-49 positions,
+		 * Geometry.h:55-65 distance2DY0, times invCT (VectorialSTAProcessor.h:241) */
+		const unsigned N = cfg->num_elements;
+		const float pitch = static_cast<float>(cfg->pitch);
+		const float xc = ((N - 1) * pitch) / 2;
+		const float fs = static_cast<float>(cfg->sampling_frequency);
+		const float c = static_cast<float>(cfg->propagation_speed);
+		const float invCT = (fs * cfg->upsampling_factor) / c;
+		const size_t npix = static_cast<size_t>(n1) * n2;
+		for (unsigned e = 0; e < N; ++e) {
+			const float xe = e * pitch - xc;
+			for (size_t pt = 0; pt < npix; ++pt) {
+				const float dx = grid_xz[2 * pt] - xe;
+				const float pz = grid_xz[2 * pt + 1];
+				delays[e * npix + pt] = std::sqrt(dx * dx + pz * pz) * invCT;
+			}
+		}
+	});
+}
+
+int oracle_das_table_f32(const oracle_sta2d* cfg, const float* analytic, uint32_t L, const float* delays,
+		uint32_t rows, uint32_t tx_row0, uint64_t npix, const float* rx_apod, const float* tx_apod, float* value,
+		float* factor, float* rxsums, int form, int nthreads)
+{
+	/* the transmit rows are checked against the table, not num_elements */
+	if (!cfg || cfg->num_elements == 0 || cfg->last_tx_elem < cfg->first_tx_elem || cfg->upsampling_factor == 0) {
+		g_error = "invalid configuration";
+		return -1;
+	}
+	if (rxsums && cfg->cf_kind == ORACLE_CF_NONE) {
+		g_error = "per-rx sums need a coherence factor";
+		return -1;
+	}
+	return guarded([&] {
+		das_table<float>(*cfg, analytic, L, delays, rows, tx_row0, npix, rx_apod, tx_apod, value, factor, form, nthreads,
+				rxsums);
 	});
 }
 

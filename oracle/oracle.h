@@ -102,6 +102,21 @@ int oracle_das_f64(const oracle_sta2d* cfg, const double* analytic, uint32_t sig
 		const float* rx_apod, const float* tx_apod,
 		double* value, double* factor, int nthreads);
 
+/* Delay-and-sum over a caller-supplied FP32 delay table delays[rows][npix]
+ * (samples; tx t at row tx_row0 + t, rx r at row r), the input the
+ * reference GPU processors use (VectorialSTACUDAProcessor.cu:30-53).  form 0
+ * keeps the CPU processor's lerp and unfused apodized sum
+ * (VectorialSTAProcessor.h:269-284); form 1 the CUDA kernels'
+ * v0 + k (v1 - v0) and sum += v * apod as fused multiply-adds
+ * (VectorialSTACUDAProcessor.cu:93-103, :163-169).  rxsums (optional, needs a
+ * coherence factor) receives the per-rx complex sums [npix][num_elements][2]. */
+/* The FP32 delay table [num_elements][n1*n2] that oracle_das_f32 computes
+ * per pixel (distance2DY0 * invCT, Geometry.h:55-65). */
+int oracle_delays_f32(const oracle_sta2d* cfg, const float* grid_xz, uint32_t n1, uint32_t n2, float* delays);
+int oracle_das_table_f32(const oracle_sta2d* cfg, const float* analytic, uint32_t signal_length,
+		const float* delays, uint32_t rows, uint32_t tx_row0, uint64_t npix, const float* rx_apod,
+		const float* tx_apod, float* value, float* factor, float* rxsums, int form, int nthreads);
+
 /* Whole frame (prepare + DAS), timing of the two stages in seconds. */
 int oracle_process_f32(const oracle_sta2d* cfg, const float* echoes,
 		const float* grid_xz, uint32_t n1, uint32_t n2,

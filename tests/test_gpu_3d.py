@@ -17,6 +17,8 @@ import oracle
 from us_lab4a_amd import imaging, workloads
 from us_lab4a_amd.sta import HipSTA3DProcessor
 
+from parity import check_table_oracle
+
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-4
@@ -38,7 +40,7 @@ def make_proc(wl, storage="f32", cf=None):
 
 
 def oracle_frame(wl, echoes, cf):
-    cfg = wl.oracle_cfg()
+    cfg = oracle.cfg3d_for(wl)
     cfg.cf_kind = {"none": oracle.CF_NONE, "phase_coherence_factor": oracle.CF_PCF,
                    "sign_coherence_factor": oracle.CF_SCF}[cf]
     a = oracle.prepare3d(cfg, echoes)
@@ -77,6 +79,7 @@ def test_small_volume_matches_oracle(small, cf):
     p = make_proc(wl, cf=cf)
     v, f = p.process_echoes(echoes)
     a_gpu = p.download_analytic()
+    d_gpu = p.download_delays()
     p.close()
     a_o, v_o, f_o, sums = oracle_frame(wl, echoes, cf)
     peak_a = float(np.abs(a_o).max())
@@ -88,12 +91,11 @@ def test_small_volume_matches_oracle(small, cf):
     if cf == "none":
         assert np.all(f == 1.0)
         return
-    kind = oracle.CF_SCF if cf == "sign_coherence_factor" else oracle.CF_PCF
-    bad = np.abs(v.astype(np.float64) * f - v_o.astype(np.float64) * f_o) > TOL * peak
-    sens = oracle.branch_sensitive(sums, kind)
-    say("CF image over the bar at", int(bad.sum()), "of", bad.size, "; branch-sensitive", int(sens.sum()))
-    assert not np.any(bad & ~sens)
-    assert bad.mean() <= 0.01
+    # value and CF image at every voxel: the oracle fed the GPU's delay table
+    # (rows: active rx, then active tx) and tensor
+    cfg = oracle.cfg2d_of_3d(oracle.cfg3d_for(wl))
+    cfg.cf_kind = {"sign_coherence_factor": oracle.CF_SCF, "phase_coherence_factor": oracle.CF_PCF}[cf]
+    check_table_oracle(v, f, cfg, a_gpu, d_gpu, wl.rx_apodization(), None, f"3-D small {cf}", tx_row0=wl.num_rx)
 
 
 @pytest.mark.parametrize("storage,shape", [("f32", 4), ("f16", 4), ("f32", 7)])
@@ -136,27 +138,28 @@ def test_full_size_volume(tmp_path):
     echoes0 = d_echoes[:1].cpu().numpy()
     del d_echoes
     a = p.download_analytic()
+    col_peak = int(np.unravel_index(np.argmax(v), v.shape)[0])
+    cols = np.unique(np.concatenate([[col_peak], np.linspace(0, v.shape[0] - 1, 7).astype(int)]))
+    pix = (cols[:, None] * wl.n_z + np.arange(wl.n_z)[None, :]).reshape(-1)
+    d_sub = p.download_delays_at(pix)
     p.close()
     # prep of transmit 0 (1024 channels) against the oracle
-    cfg1 = wl.oracle_cfg()
+    cfg1 = oracle.cfg3d_for(wl)
     cfg1.num_tx = 1
     a_o = oracle.prepare3d(cfg1, echoes0)
     assert float(np.abs(a[:1] - a_o).max()) <= 2e-6 * float(np.abs(a_o).max())
     del a_o
     # DAS of the z-columns through the brightest voxel and 7 others
-    col_peak = int(np.unravel_index(np.argmax(v), v.shape)[0])
-    cols = np.unique(np.concatenate([[col_peak], np.linspace(0, v.shape[0] - 1, 7).astype(int)]))
     g = np.ascontiguousarray(wl.grid_xyz()[cols])
     tx, rx = wl.sa3d_config().positions(0)
-    cfg = wl.oracle_cfg()
+    cfg = oracle.cfg3d_for(wl)
     v_o, f_o, sums = oracle.das3d_rxsums(cfg, a, tx, rx, g, wl.rx_apodization())
     peak = float(np.abs(v).max())
     err = float(np.abs(v[cols].astype(np.float64) - v_o).max()) / peak
     say("columns", cols.tolist(), "value error / peak", err)
     assert abs(float(np.abs(v_o).max()) - peak) <= TOL * peak   # the subset holds the volume peak
     assert err <= TOL
-    bad = np.abs(v[cols].astype(np.float64) * f[cols] - v_o.astype(np.float64) * f_o) > TOL * peak
-    sens = oracle.branch_sensitive(sums, oracle.CF_SCF)
-    say("CF over the bar", int(bad.sum()), "branch-sensitive", int(sens.sum()))
-    assert not np.any(bad & ~sens)
-    assert bad.mean() <= 0.01
+    # the same columns through the oracle fed the GPU's delay table: value and
+    # CF image at every voxel
+    check_table_oracle(v[cols], f[cols], oracle.cfg2d_of_3d(cfg), a, d_sub, wl.rx_apodization(), None,
+                       "3-D full size columns", peak=peak, tx_row0=wl.num_rx)

@@ -57,6 +57,7 @@ def test_graph_replays_match_eager_frames():
         frame(eager, ev, ef)   # other work on the device between replays
     caps, launches = g.graph_stats()
     assert caps == 1 and launches == 4   # frame 0 tunes, frame 1 is recorded, 2-5 replay
+    assert g.last_timing() == (-1.0, -1.0)   # one launch per replayed frame: no stage split
 
     # new output buffers: recorded again, same images
     v2, f2 = torch.empty(n, device="cuda"), torch.empty(n, device="cuda")

@@ -2,9 +2,6 @@
 same seeded inputs.  Bar (FP32 storage): max |value_gpu - value_oracle| <=
 1e-4 x image peak; the CF image value*factor to the same bar.  FP16/BF16
 storage: envelope error in dB against the FP64 oracle, stated per test."""
-import math
-import sys
-
 import numpy as np
 import pytest
 
@@ -15,36 +12,12 @@ from us_lab4a_amd import _lib
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-4
-
-
-def say(*a):
-    print("[gpu-test]", *a, flush=True)
-    sys.stdout.flush()
+from parity import TOL, check_reference_oracle, check_table_oracle, peak_err, say
 
 
 def torch_cuda():
     import torch
     return torch
-
-
-def peak_err(a, b):
-    return float(np.abs(a.astype(np.float64) - b.astype(np.float64)).max() / np.abs(b).max())
-
-
-def check_cf_image(v, f, v_o, f_o, rx_sums, kind, tag):
-    """CF image value*factor within TOL of the value peak, except at pixels
-    where the oracle's per-rx sums sit on a coherence-factor discontinuity
-    (oracle.branch_sensitive: the atan2 cut for PCF, a zero real part for
-    SCF), where any two FP32 summation orders may disagree.  Those pixels
-    must stay rare."""
-    peak = float(np.abs(v_o).max())
-    bad = np.abs(v.astype(np.float64) * f - v_o.astype(np.float64) * f_o) > TOL * peak
-    sens = oracle.branch_sensitive(rx_sums, kind)
-    say(tag, "CF image over the bar at", int(bad.sum()), "of", bad.size, "pixels; branch-sensitive pixels",
-        int(sens.sum()), "; unexplained", int((bad & ~sens).sum()))
-    assert not np.any(bad & ~sens)
-    assert bad.mean() <= 0.01
 
 
 def make_proc(wl, storage="f32", cf=None):
@@ -90,8 +63,8 @@ def test_prepare_matches_oracle(small, method):
     p.set_prep(method)
     p.process_echoes(echoes)
     a_gpu = p.download_analytic()
-    a32 = oracle.prepare(wl.oracle_cfg(), echoes, np.float32)
-    a64 = oracle.prepare(wl.oracle_cfg(), echoes, np.float64)
+    a32 = oracle.prepare(oracle.cfg_for(wl), echoes, np.float32)
+    a64 = oracle.prepare(oracle.cfg_for(wl), echoes, np.float64)
     m = np.abs(a64).max()
     assert np.abs(a_gpu - a64).max() / m <= 2e-6
     assert np.abs(a32 - a64).max() / m <= 2e-6
@@ -100,19 +73,19 @@ def test_prepare_matches_oracle(small, method):
 
 @pytest.mark.parametrize("cf", ["none", "phase_coherence_factor", "sign_coherence_factor"])
 def test_das_matches_oracle_on_oracle_tensor(small, cf):
+    """The oracle's FP32 tensor uploaded as is: value within 1e-4 of the peak
+    of the reference-equivalent oracle, and value and CF image within 1e-4
+    at every pixel of the oracle fed the GPU's delay table."""
     wl, echoes = small
     say("das", wl.name, cf)
-    a32 = oracle.prepare(wl.oracle_cfg(), echoes, np.float32)
+    a32 = oracle.prepare(oracle.cfg_for(wl), echoes, np.float32)
     wl_cf = workloads.sta2d_32_small()
     wl_cf.cf = cf
-    ocfg = wl_cf.oracle_cfg()
+    ocfg = oracle.cfg_for(wl_cf)
     xz = imaging.grid_xz(wl_cf.grid())
-    if cf == "none":
-        v_o, f_o = oracle.das(ocfg, a32, xz, wl_cf.rx_apodization(), wl_cf.tx_apodization())
-    else:
-        v_o, f_o, sums = oracle.das_rxsums(ocfg, a32, xz, wl_cf.rx_apodization(), wl_cf.tx_apodization())
     p = make_proc(wl_cf)
     p.upload_analytic(a32)
+    d_gpu = p.download_delays()
     import torch
     n1, n2 = wl_cf.n_x, wl_cf.n_z
     dv = torch.empty(n1 * n2, dtype=torch.float32, device="cuda")
@@ -121,32 +94,10 @@ def test_das_matches_oracle_on_oracle_tensor(small, cf):
     torch.cuda.synchronize()
     v = dv.cpu().numpy().reshape(n1, n2)
     f = df.cpu().numpy().reshape(n1, n2)
-    assert peak_err(v, v_o) <= TOL
-    if cf == "sign_coherence_factor":
-        # SCF = |1 - sqrt(1 - k^2)|^p with k = sum(sign Re)/N: recover the sign
-        # imbalance |2 n_pos - N|; equal except at single sign decisions (step 2).
-        N = wl_cf.num_elements
-
-        def imbalance(fac):
-            m = np.clip(fac.astype(np.float64), 0, 1) ** (1.0 / wl_cf.cf_param)
-            return np.rint(np.sqrt(np.clip(1.0 - (1.0 - m) ** 2, 0, 1)) * N).astype(int)
-
-        d = np.abs(imbalance(f) - imbalance(f_o))
-        say(cf, "sign imbalance equal on", float((d == 0).mean()), "of pixels; max step", int(d.max()))
-        margins = oracle.branch_margin(sums, oracle.CF_SCF)[d > 0]
-        say(cf, "sign-flip pixels", int((d > 0).sum()), "; largest branch margin among them",
-            float(margins.max()) if margins.size else None)
-        assert d.max() <= 2
-        assert not np.any((d > 0) & ~oracle.branch_sensitive(sums, oracle.CF_SCF))
-        check_cf_image(v, f, v_o, f_o, sums, oracle.CF_SCF, cf)
-    elif cf != "none":
-        check_cf_image(v, f, v_o, f_o, sums, oracle.CF_PCF, cf)
-        close = np.abs(f - f_o) <= 1e-3
-        say(cf, "factor within 1e-3 on", float(close.mean()), "of pixels")
-        assert close.mean() >= 0.99
-    else:
-        assert np.all(f == 1.0)
     p.close()
+    rx, tx = wl_cf.rx_apodization(), wl_cf.tx_apodization()
+    check_table_oracle(v, f, ocfg, a32, d_gpu, rx, tx, f"cfg1 {cf}")
+    check_reference_oracle(v, f, ocfg, a32, xz, d_gpu, rx, tx, f"cfg1 {cf}")
 
 
 @pytest.mark.parametrize("cf", ["none", "phase_coherence_factor"])
@@ -154,7 +105,7 @@ def test_full_frame_matches_oracle(small, cf):
     wl, echoes = small
     wl.cf = cf
     say("frame", wl.name, cf)
-    v_o, f_o, _, _ = oracle.process(wl.oracle_cfg(), echoes, imaging.grid_xz(wl.grid()), wl.rx_apodization(),
+    v_o, f_o, _, _ = oracle.process(oracle.cfg_for(wl), echoes, imaging.grid_xz(wl.grid()), wl.rx_apodization(),
                                      wl.tx_apodization())
     cfgs = wl.sta_config()
     grid = wl.grid()
@@ -178,9 +129,9 @@ def test_full_frame_matches_oracle(small, cf):
 def test_reduced_storage_db_error_vs_fp64(small, storage, limit_db):
     wl, echoes = small
     say("storage", storage)
-    a64 = oracle.prepare(wl.oracle_cfg(), echoes, np.float64)
+    a64 = oracle.prepare(oracle.cfg_for(wl), echoes, np.float64)
     xz = imaging.grid_xz(wl.grid())
-    v64, _ = oracle.das(wl.oracle_cfg(), a64, xz, wl.rx_apodization(), wl.tx_apodization())
+    v64, _ = oracle.das(oracle.cfg_for(wl), a64, xz, wl.rx_apodization(), wl.tx_apodization())
     p = make_proc(wl, storage=storage)
     v, _ = p.process_echoes(echoes)
     err = workloads.db_error(v, v64)
@@ -220,24 +171,25 @@ def test_graded_shape_columns_match_oracle():
     say("graded shape: tensor downloaded")
     # prep parity on a channel subset (oracle FP64)
     sub = echoes[[0, 63, 127]][:, [0, 64, 127]]
-    cfg_sub = wl.oracle_cfg()
+    cfg_sub = oracle.cfg_for(wl)
     cfg_sub.num_elements = 3
     cfg_sub.last_tx_elem = 2
     a_sub = oracle.prepare(cfg_sub, sub, np.float64)
     ref = a_gpu[[0, 63, 127]][:, [0, 64, 127]]
     assert np.abs(ref - a_sub).max() / np.abs(a_sub).max() <= 2e-6
-    # DAS parity on a column subset, oracle FP32 over the GPU tensor
+    # DAS parity on a column subset: the oracle over the GPU's tensor, with
+    # the GPU's delay table (value and CF image at every pixel) and with its
+    # own FP32 distances (value; CF image except where the distances alone
+    # move it)
     cols = [0, 137, 255, 256, 400, 511]
-    xz = imaging.grid_xz(wl.grid())[cols]
-    v_o, f_o, sums = oracle.das_rxsums(wl.oracle_cfg(), a_gpu, np.ascontiguousarray(xz), wl.rx_apodization())
-    peak = np.abs(v).max()
-    assert np.abs(v[cols] - v_o).max() / peak <= TOL
-    # CF image against the full-image peak (the column subset may miss it)
-    bad = np.abs(v[cols].astype(np.float64) * f[cols] - v_o.astype(np.float64) * f_o) > TOL * peak
-    sens = oracle.branch_sensitive(sums, oracle.CF_PCF)
-    say("graded shape: CF image over the bar at", int(bad.sum()), "pixels; unexplained", int((bad & ~sens).sum()),
-        "; branch margins of the over-bar pixels", oracle.branch_margin(sums, oracle.CF_PCF)[bad].tolist())
-    assert not np.any(bad & ~sens) and bad.mean() <= 0.01
+    xz = np.ascontiguousarray(imaging.grid_xz(wl.grid())[cols])
+    pix = (np.asarray(cols)[:, None] * wl.n_z + np.arange(wl.n_z)[None, :]).reshape(-1)
+    d_sub = p.download_delays_at(pix)
+    peak = float(np.abs(v).max())
+    cfg = oracle.cfg_for(wl)
+    check_table_oracle(v[cols], f[cols], cfg, a_gpu, d_sub, wl.rx_apodization(), None, "graded columns", peak=peak)
+    check_reference_oracle(v[cols], f[cols], cfg, a_gpu, xz, d_sub, wl.rx_apodization(), None, "graded columns",
+                           peak=peak)
     p.close()
 
 
@@ -329,7 +281,7 @@ def test_fused_prepare_shapes(shape):
     p.process_echoes(echoes)
     a_gpu = p.download_analytic()
     p.close()
-    a64 = oracle.prepare(wl.oracle_cfg(), echoes, np.float64)
+    a64 = oracle.prepare(oracle.cfg_for(wl), echoes, np.float64)
     m = np.abs(a64).max()
     err = np.abs(a_gpu - a64).max() / m
     say("fused prepare error / peak", err)
@@ -397,6 +349,8 @@ def test_tiled_kernel_over_128_transmits(cf):
     p.set_kernel(0)
     v_a, f_a = p.process_echoes(echoes)
     tuned = p.tuned_kernel()
+    a_gpu = p.download_analytic()
+    d_gpu = p.download_delays()
     p.close()
     say("autotuner chose", tuned)
     assert tuned[0] >= 0
@@ -405,17 +359,13 @@ def test_tiled_kernel_over_128_transmits(cf):
         assert peak_err(out[2][0], out[1][0]) <= TOL
     else:
         assert np.array_equal(out[1][0], out[2][0]) and np.array_equal(out[1][1], out[2][1])
-    ocfg = wl.oracle_cfg()
+    ocfg = oracle.cfg_for(wl)
     a32 = oracle.prepare(ocfg, echoes, np.float32)
     xz = imaging.grid_xz(wl.grid())
-    if cf == "none":
-        v_o, _ = oracle.das(ocfg, a32, xz, wl.rx_apodization(), wl.tx_apodization())
-    else:
-        v_o, f_o, sums = oracle.das_rxsums(ocfg, a32, xz, wl.rx_apodization(), wl.tx_apodization())
-    assert peak_err(out[2][0], v_o) <= TOL
-    if cf != "none":
-        kind = oracle.CF_PCF if cf == "phase_coherence_factor" else oracle.CF_SCF
-        check_cf_image(out[2][0], out[2][1], v_o, f_o, sums, kind, "over 128 transmits")
+    check_table_oracle(out[2][0], out[2][1], ocfg, a_gpu, d_gpu, wl.rx_apodization(), wl.tx_apodization(),
+                       f"over 128 transmits {cf}")
+    check_reference_oracle(out[2][0], out[2][1], ocfg, a32, xz, d_gpu, wl.rx_apodization(), wl.tx_apodization(),
+                           f"over 128 transmits {cf}")
 
 
 @pytest.fixture(scope="module")
@@ -429,10 +379,10 @@ def graded_fp64():
     torch.cuda.synchronize()
     echoes = d_echoes.cpu().numpy()
     say("graded FP64 oracle: preparing 16384 channels in FP64")
-    a64 = oracle.prepare(wl.oracle_cfg(), echoes, np.float64)
+    a64 = oracle.prepare(oracle.cfg_for(wl), echoes, np.float64)
     cols = [0, 137, 255, 256, 400, 511]
     xz = np.ascontiguousarray(imaging.grid_xz(wl.grid())[cols])
-    v64, f64 = oracle.das(wl.oracle_cfg(), a64, xz, wl.rx_apodization())
+    v64, f64 = oracle.das(oracle.cfg_for(wl), a64, xz, wl.rx_apodization())
     del a64
     say("graded FP64 oracle: done")
     return wl, d_echoes, cols, v64, f64

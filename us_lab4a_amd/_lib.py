@@ -183,6 +183,7 @@ SIGNATURES = {
     "mista_upload_analytic": (_i32, [_vp, _vp, _u32]),
     "mista_download_analytic": (_i32, [_vp, _vp]),
     "mista_download_delays": (_i32, [_vp, _vp]),
+    "mista_download_delays_at": (_i32, [_vp, _vp, _u64, _vp]),
     "mista_query": (_i32, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u64)]),
     "mista_set_kernel": (_i32, [_vp, _i32, _i32, _i32]),
     "mista_tuned_kernel": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),

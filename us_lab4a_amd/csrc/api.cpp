@@ -59,6 +59,7 @@ struct mista_handle {
 	// timing
 	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 	bool timed = false;
+	bool graph_frame = false;      // the last frame was a graph replay
 
 	// frame graph (mista_set_graph): the kernels of a steady device frame,
 	// captured once and replayed while nothing they depend on changes
@@ -292,7 +293,10 @@ void run_das(mista_handle* h, float* d_value, float* d_factor, hipStream_t s)
 	launch_das(h->geom, h->tuning, h->das_scratch, h->d_tensor.get(), h->d_delay.as<const float>(), h->d_w.as<const float>(),
 			d_value, d_factor, s);
 	if (!h->capturing) HIP_CHECK(hipEventRecord(h->ev[3], s));
-	h->timed = true;
+	if (!h->capturing) {   // a recording keeps the eager frame's times
+		h->timed = true;
+		h->graph_frame = false;
+	}
 }
 
 // Builds the resident delay table from h->grid_xyz and the element rows.
@@ -663,13 +667,11 @@ int mista_process_device(mista_handle* h, const float* d_echoes, uint32_t num_sa
 			return k;
 		};
 		if (h->gexec && key() == h->gkey) {
-			ensure_events(h);
-			HIP_CHECK(hipEventRecord(h->ev[0], s));
-			HIP_CHECK(hipEventRecord(h->ev[1], s));
-			HIP_CHECK(hipEventRecord(h->ev[2], s));
+			// one launch for the whole frame: the stage split is not
+			// observable (mista_last_timing reports it as unavailable)
 			HIP_CHECK(hipGraphLaunch(h->gexec, s));
-			HIP_CHECK(hipEventRecord(h->ev[3], s));
-			h->timed = true;
+			h->timed = false;
+			h->graph_frame = true;
 			++h->graph_launches;
 			return;
 		}
@@ -875,6 +877,29 @@ int mista_download_delays(mista_handle* h, float* delays)
 	});
 }
 
+int mista_download_delays_at(mista_handle* h, const uint64_t* points, uint64_t n, float* delays)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		require_grid(h);
+		if ((!points || !delays) && n) MISTA_THROW(MISTA_E_INVALID_PARAM, "null host pointer.");
+		const uint64_t npix = static_cast<uint64_t>(h->n1) * h->n2;
+		for (uint64_t i = 0; i < n; ++i)
+			if (points[i] >= npix) MISTA_THROW(MISTA_E_INVALID_PARAM, "Point index " << points[i] << " out of range.");
+		const float* d = h->d_delay.as<const float>();
+		// one copy per row and run of consecutive point indices
+		for (uint64_t i = 0; i < n;) {
+			uint64_t j = i + 1;
+			while (j < n && points[j] == points[j - 1] + 1) ++j;
+			for (uint32_t r = 0; r < h->rows; ++r)
+				HIP_CHECK(hipMemcpyAsync(delays + static_cast<uint64_t>(r) * n + i, d + static_cast<uint64_t>(r) * npix + points[i],
+						(j - i) * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+			i = j;
+		}
+		HIP_CHECK(hipStreamSynchronize(h->stream));
+	});
+}
+
 int mista_query(const mista_handle* h, uint32_t* signal_length, uint32_t* fft_size, uint64_t* num_points)
 {
 	if (!h) return MISTA_E_INVALID_PARAM;
@@ -928,7 +953,9 @@ int mista_last_timing(const mista_handle* h, float* prep_ms, float* das_ms)
 	mista_handle* hh = const_cast<mista_handle*>(h);
 	return guarded(hh, [&] {
 		float p = 0.0f, d = 0.0f;
-		if (h->timed) {
+		if (h->graph_frame) {
+			p = d = -1.0f;   // a replayed frame graph: no stage split
+		} else if (h->timed) {
 			HIP_CHECK(hipEventSynchronize(h->ev[3]));
 			if (h->ev[0] && h->ev[1] && hipEventQuery(h->ev[1]) == hipSuccess &&
 					hipEventElapsedTime(&p, h->ev[0], h->ev[1]) != hipSuccess) p = 0.0f;

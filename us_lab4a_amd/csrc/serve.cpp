@@ -263,6 +263,7 @@ bool mista_engine::admit_one(std::unique_lock<std::mutex>& lk)
 	}
 	lk.unlock();
 	int status = MISTA_OK;
+	std::string err;   // published under the lock below
 	try {
 		if (fresh) {
 			mista_sta_config c = j->spec.config;
@@ -277,13 +278,14 @@ bool mista_engine::admit_one(std::unique_lock<std::mutex>& lk)
 		launch(j, s);
 	} catch (const Error& e) {
 		status = e.code;
-		last_error = e.what();
+		err = e.what();
 	} catch (const std::exception& e) {
 		status = MISTA_E_INVALID_STATE;
-		last_error = e.what();
+		err = e.what();
 	}
 	lk.lock();
 	if (status != MISTA_OK) {
+		last_error = err;
 		j->status = status;
 		if (s.h && fresh && status != MISTA_E_HIP) {
 			// keep nothing half-built
@@ -375,7 +377,16 @@ int mista_engine_destroy(mista_engine* e)
 	return MISTA_OK;
 }
 
-const char* mista_engine_last_error(const mista_engine* e) { return e ? e->last_error.c_str() : ""; }
+const char* mista_engine_last_error(const mista_engine* e)
+{
+	// a copy taken under the engine's lock: the worker may set a new message
+	// at any time
+	thread_local std::string msg;
+	if (!e) return "";
+	std::lock_guard<std::mutex> g(const_cast<mista_engine*>(e)->mu);
+	msg = e->last_error;
+	return msg.c_str();
+}
 
 int mista_engine_submit(mista_engine* e, const mista_job* job, uint64_t* ticket)
 {

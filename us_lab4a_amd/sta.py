@@ -246,6 +246,16 @@ class HipSTAProcessor:
         check(lib().mista_download_delays(self._h, out.ctypes.data), self._h)
         return out
 
+    def _delay_rows(self):
+        return self.config.num_elements
+
+    def download_delays_at(self, points):
+        """Delay-table columns of the given point indices: [rows][len(points)]."""
+        idx = np.ascontiguousarray(points, dtype=np.uint64)
+        out = np.empty((self._delay_rows(), idx.size), np.float32)
+        check(lib().mista_download_delays_at(self._h, idx.ctypes.data, idx.size, out.ctypes.data), self._h)
+        return out
+
     def query(self):
         L, nf, npts = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
         check(lib().mista_query(self._h, ctypes.byref(L), ctypes.byref(nf), ctypes.byref(npts)), self._h)
@@ -360,6 +370,9 @@ class HipSTA3DProcessor(HipSTAProcessor):
         value, factor = self.process_echoes(echoes)
         grid_data["value"] = value
         grid_data["factor"] = factor
+
+    def _delay_rows(self):
+        return self.config.num_rx + self.config.num_tx
 
     def download_delays(self):
         n1, n2 = self._grid_shape

@@ -113,15 +113,6 @@ class Workload2D:
         return synth.echoes_gpu(tx, xyz, self.scatterers(), self.num_samples, self.fs, self.c, self.fc, self.bw,
                                 d_out, snr_db=self.snr_db, seed=self.seed_noise, stream=stream)
 
-    def oracle_cfg(self):
-        import oracle  # test infrastructure
-        cf = {"none": oracle.CF_NONE, "phase_coherence_factor": oracle.CF_PCF,
-              "sign_coherence_factor": oracle.CF_SCF}[self.cf]
-        return oracle.make_cfg(self.num_elements, self.num_samples, self.upsampling, self.pitch, self.fs, self.c,
-                               self.fc, first_tx=self.first_tx,
-                               last_tx=(self.num_elements - 1 if self.last_tx < 0 else self.last_tx),
-                               dead_zone_m=self.dead_zone_m, peak_offset=self.peak_offset, cf_kind=cf,
-                               cf_param=self.cf_param)
 
 
 def sta2d_32_small():
@@ -239,13 +230,6 @@ class Workload3D:
         return synth.echoes_gpu(tx, rx, self.scatterers(), self.num_samples, self.fs, self.c, self.fc, self.bw,
                                 d_out, snr_db=self.snr_db, seed=self.seed_noise, stream=stream)
 
-    def oracle_cfg(self):
-        import oracle  # test infrastructure
-        cf = {"none": oracle.CF_NONE, "phase_coherence_factor": oracle.CF_PCF,
-              "sign_coherence_factor": oracle.CF_SCF}[self.cf]
-        return oracle.make_cfg3d(self.num_tx, self.num_rx, self.num_samples, self.upsampling, self.fs, self.c,
-                                 self.fc, dead_zone_m=self.dead_zone_m, peak_offset=self.peak_offset, cf_kind=cf,
-                                 cf_param=self.cf_param)
 
 
 def sta3d_sparse():
