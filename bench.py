@@ -41,8 +41,10 @@ def parse():
                     help="replay steady frames as one hipGraph (stage times then cover the whole frame)")
     ap.add_argument("--tile", type=int, nargs=2, metavar=("SHAPE", "CHUNKING"), default=None,
                     help="fixed tiled-DAS parameters instead of the autotuner (profiling runs)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle leg (baseline + FP64 errors)")
+    ap.add_argument("--no-headline", action="store_true", help="skip the FP16 batched comparison")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-reps", type=int, default=5, help="timed repetitions of the CPU sample (after 1 warm-up)")
     ap.add_argument("--mode", default="frame", choices=["frame", "batch", "serve"],
                     help="frame: the headline line; batch: FP16 frames through the serving engine vs plain "
                          "FP32 frames; serve: Poisson arrivals of a mixed-shape job stream")
@@ -75,43 +77,117 @@ def measured_traffic(wl, storage):
     return None, None
 
 
-def cpu_baseline(wl, threads):
-    """Oracle (FP32 restatement of the reference TBB processor) on a bounded
-    sample of the same frame: prep of a tx subset and DAS of a column subset,
-    scaled to one frame."""
+def cpu_oracle_leg(wl, threads, reps, d_echoes, cols, images):
+    """The CPU oracle (FP32 restatement of the reference TBB processor, its own
+    FFT instead of FFTW) on this host: a bounded sample of the frame timed as
+    1 warm-up + `reps` repetitions (prep of a tx subset, DAS of a column
+    subset over all transmits), scaled to one frame; and, as the checker of
+    the GPU images, the FP64 oracle's envelope on the column subset `cols`,
+    from which the envelope error in dB of every GPU image in `images`
+    ({name: value image [n_x][n_z]}) is stated."""
+    import platform
     import oracle
-    import torch
-    from us_lab4a_amd import imaging
-    ntx_s = 64
-    ncol_s = 256
-    d = torch.empty((wl.num_tx, wl.num_elements, wl.num_samples), dtype=torch.float32, device="cuda")
-    wl.echoes_gpu(d)
-    echoes = d[:ntx_s].cpu().numpy()
-    del d
+    from us_lab4a_amd import imaging, workloads
+    ntx_s, ncol_s = 16, 32
+    echoes = d_echoes.cpu().numpy()
     cfg = oracle.cfg_for(wl)
-    cfg.last_tx_elem = cfg.first_tx_elem + ntx_s - 1
-    t0 = time.perf_counter()
-    a = oracle.prepare(cfg, echoes, np.float32, nthreads=threads)
-    t_prep = time.perf_counter() - t0
-    # DAS sample: the first ntx_s transmits over ncol_s columns spread over x
-    cols = np.linspace(0, wl.n_x - 1, ncol_s).astype(int)
-    xz = np.ascontiguousarray(imaging.grid_xz(wl.grid())[cols])
-    t0 = time.perf_counter()
-    oracle.das(cfg, a, xz, wl.rx_apodization(), None, nthreads=threads)
-    t_das = time.perf_counter() - t0
-    frac_prep = ntx_s / wl.num_tx
-    pp_sample = ncol_s * wl.n_z * ntx_s * wl.num_elements
-    t_frame = t_prep / frac_prep + t_das * (wl.pixel_pairs / pp_sample)
-    return {
-        "value": wl.pixel_pairs / t_frame / 1e9,
+    cfg_s = oracle.cfg_for(wl)
+    cfg_s.last_tx_elem = cfg_s.first_tx_elem + ntx_s - 1
+    xz_all = imaging.grid_xz(wl.grid())
+    scols = np.linspace(0, wl.n_x - 1, ncol_s).astype(int)
+    xz_s = np.ascontiguousarray(xz_all[scols])
+    t_prep, t_das = [], []
+    a_full = None
+    for k in range(reps + 1):
+        t0 = time.perf_counter()
+        oracle.prepare(cfg_s, echoes[:ntx_s], np.float32, nthreads=threads)
+        t1 = time.perf_counter()
+        if a_full is None:
+            a_full = oracle.prepare(cfg, echoes, np.float32, nthreads=threads)   # the DAS sample reads every tx
+        t2 = time.perf_counter()
+        oracle.das(cfg, a_full, xz_s, wl.rx_apodization(), wl.tx_apodization(), nthreads=threads)
+        t3 = time.perf_counter()
+        if k > 0:   # the first repetition is the warm-up
+            t_prep.append(t1 - t0)
+            t_das.append(t3 - t2)
+    del a_full
+    prep_scale = wl.num_tx / ntx_s
+    das_scale = wl.n_x / ncol_s
+    frame = np.array(t_prep) * prep_scale + np.array(t_das) * das_scale
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        model = platform.processor()
+    base = {
+        "value": wl.pixel_pairs / float(frame.mean()) / 1e9,
         "unit": "Gpixel*pair/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle FP32 restatement of VectorialSTAProcessor: prep of {ntx_s}/{wl.num_tx} tx "
-                   f"({t_prep:.2f} s) + DAS/PCF of {ncol_s}/{wl.n_x} columns x {ntx_s} tx ({t_das:.2f} s), "
-                   f"scaled to one frame = {t_frame:.1f} s"),
-        "frame_s": t_frame,
+        "sample": (f"oracle FP32 restatement of VectorialSTAProcessor (own mixed-radix FFT and direct-sum FIR, not "
+                   f"FFTW): prep of {ntx_s}/{wl.num_tx} tx and DAS+{wl.cf} of {ncol_s}/{wl.n_x} columns x all "
+                   f"{wl.num_tx} tx, 1 warm-up + {reps} repetitions, each scaled to one frame"),
+        "frame_s": {"mean": float(frame.mean()), "min": float(frame.min()), "std": float(frame.std())},
+        "prep_s_per_frame": {"mean": float(np.mean(t_prep) * prep_scale), "min": float(np.min(t_prep) * prep_scale),
+                             "std": float(np.std(t_prep) * prep_scale)},
+        "das_s_per_frame": {"mean": float(np.mean(t_das) * das_scale), "min": float(np.min(t_das) * das_scale),
+                            "std": float(np.std(t_das) * das_scale)},
+        "nproc": os.cpu_count(),
+        "cpu_model": model,
     }
+    print(f"[bench] cpu oracle frame {frame.mean():.2f} s; FP64 reference on {len(cols)} columns", file=sys.stderr,
+          flush=True)
+    a64 = oracle.prepare(cfg, echoes, np.float64, nthreads=threads)
+    v64, _ = oracle.das(cfg, a64, np.ascontiguousarray(xz_all[cols]), wl.rx_apodization(), wl.tx_apodization(),
+                        nthreads=threads)
+    del a64
+    errs = {name: round(workloads.db_error(img[cols], v64), 2) for name, img in images.items()}
+    return base, errs
+
+
+def fp16_batched(wl, d_echoes, steps, warmup, inflight):
+    """FP16 storage (per-channel scale, FP32 accumulation) with frames
+    batched through the serving engine, `inflight` frames on the device at
+    once: frames/s over `steps` frames, engine latency p50/p99, and the last
+    value image (host)."""
+    import torch
+    from us_lab4a_amd.serve import Engine, Job2D
+    from us_lab4a_amd.sta import CoherenceFactor
+    npix = wl.n_x * wl.n_z
+    outs = [(torch.empty(npix, device="cuda"), torch.empty(npix, device="cuda")) for _ in range(inflight)]
+
+    def job(k):
+        v, f = outs[k % len(outs)]
+        return Job2D(config=wl.sta_config(), upsampling=wl.upsampling, coherence=CoherenceFactor(wl.cf, wl.cf_param),
+                     grid=wl.grid(), rx_apod=wl.rx_apodization(), tx_apod=wl.tx_apodization(), echoes=d_echoes,
+                     value=v, factor=f, storage="f16", peak_offset=wl.peak_offset)
+    with Engine(max_inflight=inflight, max_queued=4 * steps + 8) as eng:
+        for t in [eng.submit(job(k)) for k in range(max(warmup, inflight))]:
+            assert eng.wait(t)["ok"]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tickets = [eng.submit(job(k)) for k in range(steps)]
+        lat = [eng.wait(t)["latency_ms"] for t in tickets]
+        torch.cuda.synchronize()
+        fps = steps / (time.perf_counter() - t0)
+    v = outs[(steps - 1) % len(outs)][0].cpu().numpy().reshape(wl.n_x, wl.n_z)
+    return {"frames_per_s": fps, "p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))}, v
+
+
+def achievable_hbm_gbs():
+    """Streaming-read rate of a 4 GiB device buffer (HIP kernel, events)."""
+    import ctypes
+    import torch
+    from us_lab4a_amd._lib import check, lib
+    buf = torch.empty(4 << 30, dtype=torch.uint8, device="cuda")
+    buf.fill_(1)
+    out = ctypes.c_double(0.0)
+    check(lib().mista_read_bandwidth_device(ctypes.c_void_p(buf.data_ptr()), buf.numel(), 10, ctypes.byref(out), None))
+    del buf
+    return out.value
 
 
 def batch_mode(args):
@@ -330,6 +406,25 @@ def main():
     achieved = alg / (das_avg * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(wl, args.storage)
     value = wl.pixel_pairs / (ms_per_step * 1e-3) / 1e9
+    cols = [0, 137, 255, 256, 400, 511] if wl.n_x == 512 else list(np.linspace(0, wl.n_x - 1, 6).astype(int))
+    images = {args.storage: d_value.cpu().numpy().reshape(wl.n_x, wl.n_z)}
+    tuned = proc.tuned_kernel()
+    proc.close()
+    headline = None
+    if not args.no_headline:
+        # FP16 storage + frames batched through the engine against this plain frame
+        print("[bench] FP16 storage, batched frames", file=sys.stderr, flush=True)
+        b16, v16 = fp16_batched(wl, d_echoes, args.steps, args.warmup, args.inflight)
+        images["f16_batched"] = v16
+        fps_plain = 1e3 / ms_per_step
+        headline = {"plain_frames_per_s": round(fps_plain, 3), "plain_storage": args.storage,
+                    "fp16_batched_frames_per_s": round(b16["frames_per_s"], 3),
+                    "fp16_batched_gpixel_pair_per_s": round(wl.pixel_pairs * b16["frames_per_s"] / 1e9, 3),
+                    "speedup": round(b16["frames_per_s"] / fps_plain, 3), "inflight": args.inflight,
+                    "fp16_batched_latency_ms": {"p50": round(b16["p50"], 3), "p99": round(b16["p99"], 3)},
+                    "target": "fp16 batched >= 1.8x the plain FP32 frames/s at <= -60 dB envelope error vs FP64"}
+    hbm = achievable_hbm_gbs()
+    print(f"[bench] achievable HBM read {hbm:.0f} GB/s", file=sys.stderr, flush=True)
     out = {
         "metric": "TFM Gpixel*pair/s (end-to-end frame: prep + DAS + PCF + envelope)",
         "value": round(value, 3),
@@ -346,7 +441,7 @@ def main():
         "config": {"workload": wl.name, "elements": wl.num_elements, "tx": wl.num_tx, "samples": wl.num_samples,
                    "upsampling": wl.upsampling, "grid": [wl.n_x, wl.n_z], "coherence": wl.cf,
                    "storage": args.storage, "pixel_pairs_per_frame": wl.pixel_pairs,
-                   "das_tile": tile_name(tuple(args.tile) if args.tile else proc.tuned_kernel()),
+                   "das_tile": tile_name(tuple(args.tile) if args.tile else tuned),
                    "graph": bool(args.graph)},
         "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                              "p99": round(float(np.percentile(lat, 99)), 4)},
@@ -360,16 +455,50 @@ def main():
                      "traffic_source": traffic_src,
                      # measured HBM-side bytes per launch over this run's launch time
                      "traffic_gbs": round(traffic / (das_avg * 1e-3) / 1e9, 1) if traffic else None,
-                     "kernel": "das (delay-and-sum + PCF + envelope)", "algorithmic_bytes": alg},
+                     "achievable_gbs": round(hbm, 1), "frac_of_achievable": round(achieved / hbm, 4),
+                     "kernel": "das (delay-and-sum + PCF + envelope)", "algorithmic_bytes": alg,
+                     "bounds": instruction_bounds(wl, args.storage, das_avg)},
     }
+    if headline:
+        out["headline"] = headline
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
-        print(f"[bench] cpu baseline on {threads} threads", file=sys.stderr, flush=True)
-        cb = cpu_baseline(wl, threads)
+        print(f"[bench] cpu oracle leg on {threads} threads", file=sys.stderr, flush=True)
+        cb, errs = cpu_oracle_leg(wl, threads, args.cpu_reps, d_echoes, cols, images)
         out["cpu_baseline"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cb.items()}
         out["speedup_vs_cpu"] = round(value / cb["value"], 1)
-    proc.close()
+        out["envelope_error_db_vs_fp64"] = dict(errs, columns=[int(c) for c in cols])
+        if headline:
+            headline["meets_target"] = bool(headline["speedup"] >= 1.8 and errs.get("f16_batched", 0.0) <= -60.0)
     print(json.dumps(out), flush=True)
+
+
+def instruction_bounds(wl, storage, das_ms):
+    """Issue-rate bounds of the DAS launch from the committed SQ counter
+    summary for this workload and storage (instructions per 64
+    pixel-pairs): the time the VALU, the scalar unit and the LDS would need
+    alone at one wave64 instruction per 4 cycles per SIMD (VALU, SALU) and
+    the LDS's cycles (instructions plus bank-conflict cycles) per CU, at
+    2.4 GHz over 256 CUs; each against the measured launch."""
+    path = os.path.join(ROOT, "profiles", f"sq_counters_{wl.name}_{storage}.md")
+    if not os.path.exists(path):
+        return None
+    per = {}
+    for line in open(path):
+        if line.startswith("per_wave_pair:"):
+            for kv in line.split(":", 1)[1].split():
+                k, v = kv.split("=")
+                per[k] = float(v)
+    if not per:
+        return None
+    wave_pairs = wl.pixel_pairs / 64.0
+    simd_cycles = 256 * 4 * 2.4e9        # SIMD issue cycles per second
+    cu_cycles = 256 * 2.4e9
+    valu_ms = wave_pairs * per["valu"] * 4 / simd_cycles * 1e3
+    salu_ms = wave_pairs * per["salu"] * 4 / simd_cycles * 1e3
+    lds_ms = wave_pairs * (per["lds"] * per.get("lds_cycles_per_inst", 4.0) + per["lds_conflict"]) / cu_cycles * 1e3
+    return {"valu_ms": round(valu_ms, 3), "salu_ms": round(salu_ms, 3), "lds_ms": round(lds_ms, 3),
+            "launch_ms": round(das_ms, 3), "source": os.path.relpath(path, ROOT), "per_64_pixel_pairs": per}
 
 
 def tile_name(tuned):

@@ -352,6 +352,12 @@ int mista_synth_echoes_device(const mista_synth_params* p,
 /* Sum of squares of a device float array (for SNR scaling), double result. */
 int mista_sum_squares_device(const float* d_x, uint64_t n, double* out, void* stream);
 
+/* Achievable HBM read rate (measurement support, no reference
+ * counterpart): times `reps` launches of a streaming-read kernel over the
+ * device buffer d_x (bytes rounded down to 16) with HIP events and returns
+ * GB/s.  Use a buffer well above the 256 MB last-level cache. */
+int mista_read_bandwidth_device(const void* d_x, uint64_t bytes, int reps, double* gb_per_s, void* stream);
+
 /* Add N(0, noise_std^2) noise from the counter-based generator to d_x. */
 int mista_add_noise_device(float* d_x, uint64_t n, float noise_std, uint64_t seed, void* stream);
 

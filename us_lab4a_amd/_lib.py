@@ -194,6 +194,7 @@ SIGNATURES = {
     "mista_synth_echoes_device": (_i32, [ctypes.POINTER(SynthParams), _vp, _vp, _vp, _vp, _vp]),
     "mista_sum_squares_device": (_i32, [_vp, _u64, ctypes.POINTER(ctypes.c_double), _vp]),
     "mista_add_noise_device": (_i32, [_vp, _u64, _f32, _u64, _vp]),
+    "mista_read_bandwidth_device": (_i32, [_vp, _u64, _i32, ctypes.POINTER(ctypes.c_double), _vp]),
 }
 
 _lib = None

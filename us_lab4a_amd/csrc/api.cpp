@@ -12,6 +12,7 @@ void synth_echoes(const mista_synth_params& p, const float* tx_xyz, const float*
 		float* d_echoes, hipStream_t s);
 double sum_squares(const float* d_x, uint64_t n, hipStream_t s);
 void add_noise(float* d_x, uint64_t n, float sigma, uint64_t seed, hipStream_t s);
+double read_bandwidth(const void* d_x, uint64_t bytes, int reps, hipStream_t s);
 }  // namespace mista
 
 using namespace mista;
@@ -981,6 +982,14 @@ int mista_sum_squares_device(const float* d_x, uint64_t n, double* out, void* st
 	return guarded(nullptr, [&] {
 		if (!d_x || !out) MISTA_THROW(MISTA_E_INVALID_PARAM, "null argument.");
 		*out = sum_squares(d_x, n, static_cast<hipStream_t>(stream));
+	});
+}
+
+int mista_read_bandwidth_device(const void* d_x, uint64_t bytes, int reps, double* gb_per_s, void* stream)
+{
+	return guarded(nullptr, [&] {
+		if (!d_x || !gb_per_s) MISTA_THROW(MISTA_E_INVALID_PARAM, "null argument.");
+		*gb_per_s = read_bandwidth(d_x, bytes, reps, static_cast<hipStream_t>(stream));
 	});
 }
 

@@ -175,18 +175,18 @@ struct DasTuning {
 struct DasScratch {
 	DeviceBuffer partial;           // small device scalars (max window slots)
 	DeviceBuffer bounds;            // per-tile element delay bounds
-	DeviceBuffer params;            // per (tile, rx, tx) window parameters
+	DeviceBuffer params;            // per (tile, rx, tx) window words
 	uint64_t bounds_version = ~0ull;
 	uint64_t params_key = 0;
 	float params_offset = 0.0f;
-	uint32_t wcap = 0;              // LDS window entries per staged pair
-	uint32_t max_slots = 0;
+	uint32_t max_slots = 0;         // 1 KiB load slots per staged window
 	uint64_t tuned_key = ~0ull;     // geometry the autotuner last chose for
 	int tuned_p0 = -1, tuned_p1 = 0;
 	int last_p0 = -1, last_p1 = 0;  // the last DAS launch (-1: gather kernel)
 };
 // Bytes of zeroed padding after the analytic tensor (window over-read).
 size_t tensor_tail_padding();
+
 void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, const void* d_tensor,
 		const float* d_delay, const float* d_w, float* d_value, float* d_factor, hipStream_t s);
 

@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(256) das_gather_kernel(const DasArgs a)
 				}
 			}
 		}
-		a.value[p] = a.signed_value ? sr * a.inv_num : sqrtf(sr * sr + si * si) * a.inv_num;
+		a.value[p] = PixelStats<MISTA_CF_NONE>::magnitude(a, sr, si);
 		if (a.factor) a.factor[p] = 1.0f;
 	} else {
 		PixelStats<CF> st;
@@ -158,25 +158,23 @@ __global__ void __launch_bounds__(256) tile_bounds_kernel(const float* __restric
 
 __global__ void __launch_bounds__(256) window_params_kernel(const float2* __restrict__ bounds, uint32_t ntiles,
 		uint32_t num_elements, uint32_t first_tx, uint32_t ntx, uint32_t ntx_pad, uint32_t nrx, uint32_t L, float offset,
-		uint32_t slot, WinParam* __restrict__ params, uint32_t* __restrict__ max_slots)
+		uint32_t esz, uint32_t* __restrict__ words, uint32_t* __restrict__ max_need)
 {
 	// Grid-stride over the entries (a capped grid): each lane folds the
-	// slot counts of many entries, then one atomic per wave.
+	// window lengths of many entries, then one atomic per wave.
 	const uint64_t n = static_cast<uint64_t>(ntiles) * nrx * ntx_pad;
-	uint32_t mslots = 0;   // every lane reaches the wave reduction below
+	uint32_t mneed = 0;   // every lane reaches the wave reduction below
 	for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
 			i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-		uint32_t slots = 0;
 		const uint32_t t = static_cast<uint32_t>(i % ntx_pad);
 		const uint32_t r = static_cast<uint32_t>((i / ntx_pad) % nrx);
 		const uint32_t tile = static_cast<uint32_t>(i / (static_cast<uint64_t>(ntx_pad) * nrx));
-		uint32_t start = 0, word = kEdgeBit;   // padded transmit or nothing to stage
+		uint32_t word = kEdgeBit;   // padded transmit or nothing to stage: the channel's start, excluded
 		if (t < ntx) {
 			const float2 bt = bounds[static_cast<uint64_t>(tile) * num_elements + first_tx + t];
 			const float2 br = bounds[static_cast<uint64_t>(tile) * num_elements + r];
 			const float lo = (offset + bt.x) + br.x;
 			const float hi = (offset + bt.y) + br.y;
-			const uint32_t chan = t * nrx + r;
 			// lo past the record end or NaN: nothing staged, every lane of
 			// the pair is excluded by the predicated path.  (lo is never
 			// negative: delays are >= 0 and the peak offset is >= 0.)
@@ -184,20 +182,17 @@ __global__ void __launch_bounds__(256) window_params_kernel(const float2* __rest
 				const uint32_t ws = static_cast<uint32_t>(floorf(lo));
 				const float fhi = floorf(hi);
 				// entries [ws, floor(hi) + 1], never past index L (one past the record)
-				int need = (fhi >= static_cast<float>(L)) ? static_cast<int>(L) + 1 - static_cast<int>(ws)
-						: static_cast<int>(fhi) + 2 - static_cast<int>(ws);
-				if (need > static_cast<int>(L + 1 - ws)) need = static_cast<int>(L + 1 - ws);
-				slots = (static_cast<uint32_t>(need) + slot - 1) / slot;
+				uint32_t need = (fhi >= static_cast<float>(L)) ? L + 1u - ws : static_cast<uint32_t>(fhi) + 2u - ws;
+				need = min(need, L + 1u - ws);
+				mneed = max(mneed, need);
 				const bool edge = fhi + 1.0f >= static_cast<float>(L - 1u);
-				word = ws | (min(slots, 511u) << 21) | (edge ? kEdgeBit : 0u);
-				start = chan * L + ws;
+				word = ws * esz | (edge ? kEdgeBit : 0u);
 			}
 		}
-		params[i] = WinParam{start, word};
-		mslots = max(mslots, slots);
+		words[i] = word;
 	}
-	for (int o = 32; o > 0; o >>= 1) mslots = max(mslots, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mslots), o, 64)));
-	if ((threadIdx.x & 63) == 0 && mslots) atomicMax(max_slots, mslots);
+	for (int o = 32; o > 0; o >>= 1) mneed = max(mneed, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mneed), o, 64)));
+	if ((threadIdx.x & 63) == 0 && mneed) atomicMax(max_need, mneed);
 }
 
 }  // namespace
@@ -211,9 +206,21 @@ __global__ void __launch_bounds__(256) window_params_kernel(const float2* __rest
 // smallest of 52 (three workgroups per CU: the kernel is compiled for three
 // waves per SIMD), 76 (two) and 150 (one) in which the windows fit).
 // Returns false when the tables do not fit the tiled kernel.
-bool launch_tiled(const FrameGeom& g, int p0, int p1, DasScratch& scratch, const DasArgs& a, const float* d_delay,
-		hipStream_t s)
+namespace {
+
+__global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
+
+// Tile shape p0 (see launch_tiled) of a geometry.
+struct TilePlan {
+	TileGeom tg;
+	uint32_t ntiles = 0, nt = 0, nw = 0, esz = 0;
+	uint64_t key = 0;        // what the window tables depend on, besides grid and offset
+	bool ok = false;
+};
+
+TilePlan tile_plan(const FrameGeom& g, int p0)
 {
+	TilePlan tp;
 	const uint32_t ny = g.ny ? g.ny : 1u;
 	uint32_t tw1 = 16, tw2 = 1, tzh = 16;
 	if (ny > 1) { tw1 = 8; tw2 = 8; tzh = 4; }
@@ -223,57 +230,80 @@ bool launch_tiled(const FrameGeom& g, int p0, int p1, DasScratch& scratch, const
 	else if (p0 == 5) { tw1 = 16; tw2 = 1; tzh = 32; }
 	else if (p0 == 6) { tw1 = 32; tw2 = 1; tzh = 16; }
 	else if (p0 == 7) { tw1 = 8; tw2 = 8; tzh = 8; }
-	const uint32_t nt = tw1 * tw2 * tzh, nw = nt / 64;
-	if (g.ntx_pad > 128 && nt != 256) return false;   // several transmit blocks: 256-lane tiles only
-	const uint32_t esz = static_cast<uint32_t>(storage_bytes(g.storage));
-	const uint32_t slot = 64 * (16 / esz);
+	tp.nt = tw1 * tw2 * tzh;
+	tp.nw = tp.nt / 64;
+	if (g.ntx_pad > 128 && tp.nt != 256) return tp;   // several transmit blocks: 256-lane tiles only
+	tp.esz = static_cast<uint32_t>(storage_bytes(g.storage));
 	const uint32_t nx = g.n1 / ny;
-	TileGeom tg{g.n1, g.n2, ny, nx, tw1, tw2, tzh, (nx + tw1 - 1) / tw1, (ny + tw2 - 1) / tw2, (g.n2 + tzh - 1) / tzh};
-	const uint64_t ntiles64 = static_cast<uint64_t>(tg.tiles_x) * tg.tiles_y * tg.tiles_z;
-	if (ntiles64 >= (1ull << 31)) return false;
-	const uint32_t ntiles = static_cast<uint32_t>(ntiles64);
-	const uint64_t key = (static_cast<uint64_t>(tw1) << 56) ^ (static_cast<uint64_t>(tw2) << 52) ^
+	tp.tg = TileGeom{g.n1, g.n2, ny, nx, tw1, tw2, tzh, (nx + tw1 - 1) / tw1, (ny + tw2 - 1) / tw2, (g.n2 + tzh - 1) / tzh};
+	const uint64_t ntiles64 = static_cast<uint64_t>(tp.tg.tiles_x) * tp.tg.tiles_y * tp.tg.tiles_z;
+	if (ntiles64 >= (1ull << 31)) return tp;
+	tp.ntiles = static_cast<uint32_t>(ntiles64);
+	tp.key = (static_cast<uint64_t>(tw1) << 56) ^ (static_cast<uint64_t>(tw2) << 52) ^
 			(static_cast<uint64_t>(tzh) << 44) ^ (static_cast<uint64_t>(ny) << 40) ^
-			(static_cast<uint64_t>(esz) << 36) ^ (static_cast<uint64_t>(g.L) << 8) ^ g.first_tx ^
+			(static_cast<uint64_t>(tp.esz) << 36) ^ (static_cast<uint64_t>(g.L) << 8) ^ g.first_tx ^
 			(static_cast<uint64_t>(g.ntx) << 20) ^ (static_cast<uint64_t>(g.ntx_pad) << 28);
-	if (scratch.bounds_version != g.grid_version || scratch.params_key != key || scratch.params_offset != g.offset) {
-		float2* b = static_cast<float2*>(scratch.bounds.reserve(static_cast<size_t>(ntiles) * g.num_elements * sizeof(float2)));
-		hipLaunchKernelGGL(tile_bounds_kernel, dim3(ntiles), dim3(256), 0, s, d_delay, g.npix, g.num_elements, tg, b);
-		HIP_CHECK(hipGetLastError());
-		const uint64_t np = static_cast<uint64_t>(ntiles) * g.nrx * g.ntx_pad;
-		WinParam* prm = static_cast<WinParam*>(scratch.params.reserve(np * sizeof(WinParam)));
-		uint32_t* d_max = static_cast<uint32_t*>(scratch.partial.reserve(256));
-		HIP_CHECK(hipMemsetAsync(d_max, 0, sizeof(uint32_t), s));
-		hipLaunchKernelGGL(window_params_kernel, dim3(static_cast<uint32_t>(std::min<uint64_t>((np + 255) / 256, 4096))),
-				dim3(256), 0, s, b, ntiles,
-				g.num_elements, g.first_tx, g.ntx, g.ntx_pad, g.nrx, g.L, g.offset, slot, prm, d_max);
-		HIP_CHECK(hipGetLastError());
-		uint32_t max_slots = 0;
-		HIP_CHECK(hipMemcpyAsync(&max_slots, d_max, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-		HIP_CHECK(hipStreamSynchronize(s));
-		scratch.wcap = std::max(1u, max_slots) * slot;
-		scratch.max_slots = max_slots;
-		scratch.bounds_version = g.grid_version;
-		scratch.params_key = key;
-		scratch.params_offset = g.offset;
-	}
-	if (scratch.max_slots > 511) return false;
+	tp.ok = true;
+	return tp;
+}
+
+size_t bounds_bytes(const FrameGeom& g, const TilePlan& tp)
+{
+	return static_cast<size_t>(tp.ntiles) * g.num_elements * sizeof(float2);
+}
+size_t words_bytes(const FrameGeom& g, const TilePlan& tp)
+{
+	return static_cast<size_t>(tp.ntiles) * g.nrx * g.ntx_pad * sizeof(uint32_t);
+}
+
+// Builds the per-tile delay bounds and the window words; returns the window
+// size in 1 KiB slots (reads back the longest window: synchronises s).
+uint32_t build_window_tables(const FrameGeom& g, const TilePlan& tp, const float* d_delay, float2* bounds,
+		uint32_t* words, uint32_t* d_max, hipStream_t s)
+{
+	hipLaunchKernelGGL(tile_bounds_kernel, dim3(tp.ntiles), dim3(256), 0, s, d_delay, g.npix, g.num_elements, tp.tg,
+			bounds);
+	HIP_CHECK(hipGetLastError());
+	const uint64_t np = static_cast<uint64_t>(tp.ntiles) * g.nrx * g.ntx_pad;
+	hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, d_max);
+	HIP_CHECK(hipGetLastError());
+	hipLaunchKernelGGL(window_params_kernel, dim3(static_cast<uint32_t>(std::min<uint64_t>((np + 255) / 256, 4096))),
+			dim3(256), 0, s, bounds, tp.ntiles, g.num_elements, g.first_tx, g.ntx, g.ntx_pad, g.nrx, g.L, g.offset,
+			tp.esz, words, d_max);
+	HIP_CHECK(hipGetLastError());
+	uint32_t max_need = 0;
+	HIP_CHECK(hipMemcpyAsync(&max_need, d_max, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+	HIP_CHECK(hipStreamSynchronize(s));
+	const uint32_t slot = 1024u / tp.esz;   // entries per 1 KiB load slot
+	return std::max(1u, (max_need + slot - 1) / slot);
+}
+
+// Chunking and launch of the tiled kernel over window tables with S slots.
+// p1: byte 0 pairs per chunk (4 or 8), byte 1 chunk buffers, byte 2 LDS
+// budget in KiB (default: the smallest of 52 (three workgroups per CU: the
+// kernel is compiled for three waves per SIMD), 76 (two) and 150 (one) in
+// which the windows fit).  Returns false when nothing fits.
+bool launch_tiles(const FrameGeom& g, const TilePlan& tp, int p0, int p1, uint32_t S, const uint32_t* words,
+		const DasArgs& a, DasScratch& scratch, hipStream_t s)
+{
+	if (S > kMaxSlots) return false;
+	const uint32_t nt = tp.nt, nw = tp.nw;
 	// Pairs per chunk (tc) and chunk buffers in the LDS ring (nbuf): the most
 	// pairs in flight, (nbuf - 1) * tc, that fit the budget and the 6-bit
 	// vmcnt of the per-chunk waits.
-	const uint32_t S = scratch.wcap / slot;
 	const uint32_t want_tc = static_cast<uint32_t>(p1) & 0xFF, want_nb = (static_cast<uint32_t>(p1) >> 8) & 0xFF;
 	const uint32_t want_kb = (static_cast<uint32_t>(p1) >> 16) & 0xFF;
 	uint32_t tc = 0, nbuf = 0;
 	for (uint32_t kb : {52u, 76u, 150u}) {
 		if (want_kb && kb != 52u) break;
-		const size_t budget = static_cast<size_t>(want_kb ? std::min(want_kb, 158u) : kb) * 1024;
+		const size_t budget = static_cast<size_t>(want_kb ? std::min(want_kb, 158u) : kb) * 1024 -
+				tile_stats_bytes(g.cf_kind, nt);
 		auto fit = [&](uint32_t c) -> uint32_t {
-			const size_t bb = static_cast<size_t>(c) * scratch.wcap * esz + nt * 4;
+			if (c % nw != 0) return 0u;   // every wave copies whole windows
+			const size_t bb = tile_buffer_bytes(c, S, nt);
 			uint32_t nb = static_cast<uint32_t>(std::min<size_t>(16, budget / bb));
 			if (want_nb) nb = std::min(nb, want_nb);
-			if ((c * S) % nw != 0) return 0u;   // every wave must issue the same number of copies
-			const uint32_t per = c * S / nw + 1;   // copies per wave per chunk, at most
+			const uint32_t per = c / nw * S + 1;   // copies per wave per chunk, at most
 			while (nb > 2 && (nb - 2) * per > 63) --nb;
 			return (nb >= 2 && 2 * per <= 63) ? nb : 0u;
 		};
@@ -289,29 +319,50 @@ bool launch_tiled(const FrameGeom& g, int p0, int p1, DasScratch& scratch, const
 	if (nbuf == 0) return false;
 	TileArgs A;
 	A.d = a;
-	A.tg = tg;
-	A.params = scratch.params.as<const WinParam>();
-	A.wcap = scratch.wcap;
+	A.tg = tp.tg;
+	A.words = words;
+	A.S = S;
 	A.ntx_pad = g.ntx_pad;
 	A.nbuf = nbuf;
-	scratch.last_p0 = p0 == 0 ? (ny > 1 ? 4 : 1) : p0;
+	scratch.last_p0 = p0 == 0 ? (tp.tg.ny > 1 ? 4 : 1) : p0;
 	scratch.last_p1 = static_cast<int>(tc | (nbuf << 8) | (want_kb << 16));
 	switch (g.storage) {
-	case MISTA_STORAGE_F32: launch_tile_storage<MISTA_STORAGE_F32>(g.cf_kind, A, ntiles, tc, s); break;
-	case MISTA_STORAGE_F16: launch_tile_storage<MISTA_STORAGE_F16>(g.cf_kind, A, ntiles, tc, s); break;
-	case kStorageReal: launch_tile_storage<kStorageReal>(g.cf_kind, A, ntiles, tc, s); break;
-	default: launch_tile_storage<MISTA_STORAGE_BF16>(g.cf_kind, A, ntiles, tc, s); break;
+	case MISTA_STORAGE_F32: launch_tile_storage<MISTA_STORAGE_F32>(g.cf_kind, A, tp.ntiles, tc, s); break;
+	case MISTA_STORAGE_F16: launch_tile_storage<MISTA_STORAGE_F16>(g.cf_kind, A, tp.ntiles, tc, s); break;
+	case kStorageReal: launch_tile_storage<kStorageReal>(g.cf_kind, A, tp.ntiles, tc, s); break;
+	default: launch_tile_storage<MISTA_STORAGE_BF16>(g.cf_kind, A, tp.ntiles, tc, s); break;
 	}
 	HIP_CHECK(hipGetLastError());
 	return true;
 }
 
-// Tail padding of the analytic tensor (bytes): staged windows may read up to
-// one load slot past the last sample of the last channel.
-size_t tensor_tail_padding() { return 64 * 16; }
+}  // namespace
 
-void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, const void* d_tensor,
-		const float* d_delay, const float* d_w, float* d_value, float* d_factor, hipStream_t s)
+// Builds (or reuses) the window tables for tile shape p0 and launches the
+// tiled kernel with the chunking p1.  p0: 0/1 = 16x16 (planes) or 8x8x4
+// (volumes), 2 = 8x32, 3 = 32x8, 4 = 8x8x4; 512-point tiles (512-lane
+// workgroups, 8 pairs per chunk): 5 = 16x32, 6 = 32x16, 7 = 8x8x8.  p1: see
+// launch_tiles.  Returns false when the tables do not fit the tiled kernel.
+bool launch_tiled(const FrameGeom& g, int p0, int p1, DasScratch& scratch, const DasArgs& a, const float* d_delay,
+		hipStream_t s)
+{
+	const TilePlan tp = tile_plan(g, p0);
+	if (!tp.ok) return false;
+	if (scratch.bounds_version != g.grid_version || scratch.params_key != tp.key || scratch.params_offset != g.offset) {
+		float2* b = static_cast<float2*>(scratch.bounds.reserve(bounds_bytes(g, tp)));
+		uint32_t* words = static_cast<uint32_t*>(scratch.params.reserve(words_bytes(g, tp)));
+		uint32_t* d_max = static_cast<uint32_t*>(scratch.partial.reserve(256));
+		scratch.max_slots = build_window_tables(g, tp, d_delay, b, words, d_max, s);
+		scratch.bounds_version = g.grid_version;
+		scratch.params_key = tp.key;
+		scratch.params_offset = g.offset;
+	}
+	return launch_tiles(g, tp, p0, p1, scratch.max_slots, scratch.params.as<const uint32_t>(), a, scratch, s);
+}
+
+// Kernel arguments of a frame.
+DasArgs das_args(const FrameGeom& g, const void* d_tensor, const float* d_delay, const float* d_w, float* d_value,
+		float* d_factor)
 {
 	DasArgs a;
 	a.tensor = d_tensor;
@@ -336,10 +387,22 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 	}
 	a.inv_num = 1.0f / static_cast<float>(static_cast<uint64_t>(g.ntx) * g.nrx);
 	a.signed_value = (g.storage == kStorageReal) ? 1 : 0;
+	return a;
+}
 
-	// window start entries are u32 (channel * L + ws): tensor below 2^32 entries
-	const bool can_tile = (t.variant != 1) && (g.ntx_pad <= 128 || g.ntx_pad % 128 == 0) && g.L < (1u << 21) - 64 &&
-			static_cast<uint64_t>(g.ntx) * g.nrx * g.L < (1ull << 32) - (1ull << 22);
+// Tail padding of the analytic tensor (bytes): a staged window is copied as
+// whole 1 KiB slots from its first entry, so the last channel's copies may
+// read up to kMaxSlots KiB past its last sample.
+size_t tensor_tail_padding() { return static_cast<size_t>(kMaxSlots) * 1024; }
+
+void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, const void* d_tensor,
+		const float* d_delay, const float* d_w, float* d_value, float* d_factor, hipStream_t s)
+{
+	const DasArgs a = das_args(g, d_tensor, d_delay, d_w, d_value, d_factor);
+
+	// window words hold ws * ESZ in 31 bits (L <= 2^20 samples)
+	const bool can_tile = (t.variant != 1) && (g.ntx_pad <= 128 || g.ntx_pad % 128 == 0) &&
+			static_cast<uint64_t>(g.L) * storage_bytes(g.storage) < (1ull << 31);
 	if (can_tile) {
 		if (t.variant == 2 || t.p0 || t.p1) {
 			if (launch_tiled(g, t.p0, t.p1, scratch, a, d_delay, s)) return;

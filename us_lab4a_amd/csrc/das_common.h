@@ -31,9 +31,19 @@ template<int STORAGE> struct Entry;
 // get: one entry as FP32 (re, im).  lerp: v0 + k (v1 - v0) per component
 // for the entries e[0], e[1], with FP32 arithmetic: the difference rounded
 // once, then one fused multiply-add.
+// lerp2: the same from the raw words of the two entries as one LDS read
+// returns them (LdsPair).
+template<uint32_t ESZ> struct LdsPair;
+template<> struct LdsPair<8> { using T = float4; };   // two 8-byte entries
+template<> struct LdsPair<4> { using T = uint2; };    // two 4-byte entries
+
 template<> struct Entry<MISTA_STORAGE_F32> {
 	using T = float2;
 	static __device__ __forceinline__ float2 get(T v) { return v; }
+	static __device__ __forceinline__ float2 lerp2(float4 e, float k)
+	{
+		return make_float2(fmaf(k, e.z - e.x, e.x), fmaf(k, e.w - e.y, e.y));
+	}
 	static __device__ __forceinline__ float2 lerp(const T* e, float k)
 	{
 		const float2 v0 = e[0], v1 = e[1];
@@ -56,6 +66,11 @@ template<> struct Entry<MISTA_STORAGE_F16> {
 		const float2 v0 = __half22float2(h0);
 		return make_float2(fmaf(k, dx, v0.x), fmaf(k, dy, v0.y));
 	}
+	static __device__ __forceinline__ float2 lerp2(uint2 e, float k)
+	{
+		T h[2] = {__builtin_bit_cast(T, e.x), __builtin_bit_cast(T, e.y)};
+		return lerp(h, k);
+	}
 };
 template<> struct Entry<MISTA_STORAGE_BF16> {
 	using T = uint32_t;
@@ -68,6 +83,11 @@ template<> struct Entry<MISTA_STORAGE_BF16> {
 		const float2 v0 = get(e[0]), v1 = get(e[1]);
 		return make_float2(fmaf(k, v1.x - v0.x, v0.x), fmaf(k, v1.y - v0.y, v0.y));
 	}
+	static __device__ __forceinline__ float2 lerp2(uint2 e, float k)
+	{
+		const T h[2] = {e.x, e.y};
+		return lerp(h, k);
+	}
 };
 
 template<> struct Entry<kStorageReal> {
@@ -77,6 +97,11 @@ template<> struct Entry<kStorageReal> {
 	{
 		const float v0 = e[0], v1 = e[1];
 		return make_float2(fmaf(k, v1 - v0, v0), 0.0f);
+	}
+	static __device__ __forceinline__ float2 lerp2(uint2 e, float k)
+	{
+		const T h[2] = {__uint_as_float(e.x), __uint_as_float(e.y)};
+		return lerp(h, k);
 	}
 };
 
@@ -104,7 +129,7 @@ struct Moments {
 		if (first) k = x;
 		const double d = static_cast<double>(x) - k;
 		s1 += d;
-		s2 += d * d;
+		s2 = fma(d, d, s2);
 	}
 	__device__ float sigma(uint32_t n) const
 	{
@@ -134,30 +159,100 @@ struct PixelStats {
 			sign_sum += (ar >= 0.0f) ? 1 : -1;
 		}
 	}
+	// Multiply-adds are written out (fmaf) wherever the compiler could
+	// otherwise contract differently in the gather and the tiled kernel.
 	__device__ __forceinline__ void write(const DasArgs& a, uint64_t p) const
 	{
-		a.value[p] = a.signed_value ? sr * a.inv_num : sqrtf(sr * sr + si * si) * a.inv_num;
+		a.value[p] = magnitude(a, sr, si);
 		if (!a.factor) return;
 		float f = 1.0f;
 		if constexpr (CF == MISTA_CF_PCF) {
 			const float sf = fminf(m0.sigma(a.nrx), m1.sigma(a.nrx));
-			f = fmaxf(0.0f, 1.0f - a.cf_factor * sf);
+			f = fmaxf(0.0f, fmaf(-a.cf_factor, sf, 1.0f));
 		} else if constexpr (CF == MISTA_CF_SCF) {
 			const float k = static_cast<float>(sign_sum) / static_cast<float>(a.nrx);
-			const float mm = 1.0f - sqrtf(1.0f - k * k);
+			const float mm = 1.0f - sqrtf(fmaf(-k, k, 1.0f));
 			f = powf(fabsf(mm), a.cf_factor);
 		}
 		a.factor[p] = f;
 	}
+	// value = |sum| / (Ntx * Nrx), or the signed sum for real samples
+	static __device__ __forceinline__ float magnitude(const DasArgs& a, float re, float im)
+	{
+		return a.signed_value ? re * a.inv_num : sqrtf(fmaf(re, re, im * im)) * a.inv_num;
+	}
 };
 
+// Per-rx statistics of the tiled kernel: PixelStats with the PCF moments (4
+// doubles per lane) kept in LDS at m.  They change once per rx, and in
+// registers they would push the kernel past its register budget (128
+// registers hold the lane's transmit delays).  The arithmetic is
+// PixelStats's, so both kernels give the same bits.
+template<int CF>
+struct TileStats {
+	float sr = 0.0f, si = 0.0f;
+	float k0 = 0.0f, k1 = 0.0f;   // the moments' shifts (first phase, first aux)
+	int sign_sum = 0;
+	double* m = nullptr;          // LDS: s1 and s2 of the phases, then of the aux phases
+	__device__ __forceinline__ void add_rx(float ar, float ai, bool first)
+	{
+		sr += ar;
+		si += ai;
+		if constexpr (CF == MISTA_CF_PCF) {
+			const float phi = atan2f(ai, ar);
+			const float aux = phi + ((phi < 0.0f) ? kPi : -kPi);
+			if (first) {
+				k0 = phi;
+				k1 = aux;
+			}
+			Moments m0, m1;
+			m0.k = k0;
+			m1.k = k1;
+			if (!first) {
+				m0.s1 = m[0];
+				m0.s2 = m[1];
+				m1.s1 = m[2];
+				m1.s2 = m[3];
+			}
+			m0.add(phi, false);
+			m1.add(aux, false);
+			m[0] = m0.s1;
+			m[1] = m0.s2;
+			m[2] = m1.s1;
+			m[3] = m1.s2;
+		} else if constexpr (CF == MISTA_CF_SCF) {
+			sign_sum += (ar >= 0.0f) ? 1 : -1;
+		}
+	}
+	__device__ __forceinline__ void write(const DasArgs& a, uint64_t p) const
+	{
+		PixelStats<CF> ps;
+		ps.sr = sr;
+		ps.si = si;
+		ps.sign_sum = sign_sum;
+		if constexpr (CF == MISTA_CF_PCF) {
+			ps.m0.k = k0;
+			ps.m1.k = k1;
+			ps.m0.s1 = m[0];
+			ps.m0.s2 = m[1];
+			ps.m1.s1 = m[2];
+			ps.m1.s2 = m[3];
+		}
+		ps.write(a, p);
+	}
+};
+// LDS bytes of the tiled kernel's per-lane statistics
+inline size_t tile_stats_bytes(int cf, uint32_t nt) { return cf == MISTA_CF_PCF ? static_cast<size_t>(nt) * 32 : 0; }
+
 // Tile geometry: a tw1 (x) by tw2 (y) by tzh (z) block of grid points, one
-// point per lane, x fastest within a wave (neighbouring lanes read nearby
-// window entries: few LDS conflicts).  A plane has ny = tw2 = 1; a volume is
-// laid out as n1 = nx * ny columns (y fastest) by n2 = nz rows.  Tiles are
-// numbered depth-band major (all tiles of a band of tzh rows are
-// consecutive), so the workgroups resident at one time cover a depth band and
-// share signal windows through L2/MALL.
+// point per lane.  A plane has ny = tw2 = 1; a volume is laid out as
+// n1 = nx * ny columns (y fastest) by n2 = nz rows.  In a plane tile each wave
+// takes an 8 x 8 block of points (x fastest within the block): the lanes of
+// one LDS read pass then sit close together in both directions, which spreads
+// their window entries over more banks than a 16 x 4 strip.  Volume tiles
+// keep x fastest, then y, then z.  Tiles are numbered depth-band major (all
+// tiles of a band of tzh rows are consecutive), so the workgroups resident
+// at one time cover a depth band and share signal windows through L2/MALL.
 struct TileGeom {
 	uint32_t n1, n2, ny, nx;
 	uint32_t tw1, tw2, tzh;
@@ -167,7 +262,17 @@ struct TileGeom {
 		const uint32_t per_band = tiles_x * tiles_y;
 		const uint32_t tz = tile / per_band, rem = tile % per_band;
 		const uint32_t tx = rem / tiles_y, ty = rem % tiles_y;
-		const uint32_t lx = lane_id % tw1, ly = (lane_id / tw1) % tw2, lz = lane_id / (tw1 * tw2);
+		uint32_t lx, ly, lz;
+		if (tw2 == 1 && tw1 % 8 == 0 && tzh % 8 == 0) {
+			const uint32_t blk = lane_id / 64, l = lane_id % 64, bx = tw1 / 8;
+			lx = (blk % bx) * 8 + l % 8;
+			lz = (blk / bx) * 8 + l / 8;
+			ly = 0;
+		} else {
+			lx = lane_id % tw1;
+			ly = (lane_id / tw1) % tw2;
+			lz = lane_id / (tw1 * tw2);
+		}
 		const uint32_t ix = tx * tw1 + lx, iy = ty * tw2 + ly, iz = tz * tzh + lz;
 		valid = ix < nx && iy < ny && iz < n2;
 		// lanes outside the grid shadow an in-tile point
@@ -176,37 +281,40 @@ struct TileGeom {
 	}
 };
 
-// Window parameters per (tile, rx, tx), packed in a u32:
-//   bits  0..20  ws: first sample of the window (floor of the tile's lowest position)
-//   bits 21..29  number of SLOT-entry load slots to stage
-//   bit  31      record-end flag: some lane may have i + 1 >= L (or pos < 0)
-// lo/hi use the same float expression order as the positions,
-// (offset + d_tx) + d_rx, so by monotonic rounding every lane's floor(pos)
-// lies in [ws, floor(hi)] and floor(pos) + 1 in the staged window.
-struct WinParam {
-	uint32_t start;   // absolute entry index of the window start (channel * L + ws)
-	uint32_t word;    // ws | slots << 21 | edge << 31
-};
-constexpr uint32_t kWsMask = (1u << 21) - 1;
+// Window word per (tile, rx, tx):
+//   bits 0..30  ws * ESZ: byte offset in the channel of the first staged entry
+//               (ws = floor of the tile's lowest position for the pair)
+//   bit  31     record-end flag: some lane may have pos < 0 or i + 1 >= L,
+//               or the pair has nothing to stage (padded transmit, window
+//               past the record end)
+// The lowest and highest positions use the same float expression order as
+// the lanes', (offset + d_tx) + d_rx, so by monotonic rounding every lane's
+// floor(pos) lies in [ws, floor(hi)], and floor(pos) + 1 in the staged
+// window of wcap entries.  The channel itself (t * nrx + r) is not stored:
+// the kernel forms its 64-bit address, so tensors of any size are tiled.
 constexpr uint32_t kEdgeBit = 1u << 31;
 
 typedef __attribute__((address_space(1))) const void* gptr_t;
 typedef __attribute__((address_space(3))) void* lptr_t;
-// Read-only tables (window parameters, weights) through the constant
-// address space: uniform indices then become scalar loads.
-typedef __attribute__((address_space(4))) const WinParam* cu2_t;
+// Read-only tables (window words, weights) through the constant address
+// space: uniform indices then become scalar loads.
+typedef __attribute__((address_space(4))) const uint32_t* cu32_t;
 typedef __attribute__((address_space(4))) const float* cf32_t;
 
 struct TileArgs {
 	DasArgs d;
 	TileGeom tg;
-	const WinParam* params;   // [ntiles][nrx][ntx_pad]
-	uint32_t wcap;            // LDS window capacity per pair (entries, multiple of SLOT)
+	const uint32_t* words;    // [ntiles][nrx][ntx_pad]
+	uint32_t S;               // 1 KiB load slots per staged window (wcap = S * SLOT entries)
 	uint32_t ntx_pad;         // transmits padded to the instantiated count
 	uint32_t nbuf;            // chunk buffers in the LDS ring (>= 2)
 };
 
 constexpr uint32_t kTileThreads = 256;
+// Zeroed bytes after the analytic tensor: a window is staged as whole 1 KiB
+// slots from its first entry, so the last channel's copies may read up to
+// (slots per window) KiB past its end; the launcher keeps windows below this.
+constexpr uint32_t kMaxSlots = 64;
 
 // Direct global -> LDS copies (LDS DMA).  Each lane moves 16 (4) bytes from
 // its own address to lds + 16 (4) * lane; lds must be wave-uniform.  Written
@@ -217,13 +325,38 @@ constexpr uint32_t kTileThreads = 256;
 // kernels using these never rely on m0 themselves.
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
-__device__ __forceinline__ void lds_dma16(const void* g, uint32_t lds)
-{
-	asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" :: "s"(lds), "v"(g) : "memory", "m0");
-}
 __device__ __forceinline__ void lds_dma4(const void* g, uint32_t lds)
 {
 	asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off" :: "s"(lds), "v"(g) : "memory", "m0");
+}
+// n (1..4) consecutive 1 KiB slots: global g + k KiB -> LDS lds + k KiB.
+// The instruction offset of global_load_lds moves both the global and the
+// LDS address (measured on gfx950: offset:1024 with m0 = B lands the source
+// bytes [1024, 2048) at LDS B + 1024), so one m0 serves four slots.
+__device__ __forceinline__ void lds_dma16_slots(const void* g, uint32_t lds, uint32_t n)
+{
+	switch (n) {
+	case 1:
+		asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+				:: "s"(lds), "v"(g) : "memory", "m0");
+		break;
+	case 2:
+		asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off\n\t"
+				"global_load_lds_dwordx4 %1, off offset:1024"
+				:: "s"(lds), "v"(g) : "memory", "m0");
+		break;
+	case 3:
+		asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off\n\t"
+				"global_load_lds_dwordx4 %1, off offset:1024\n\tglobal_load_lds_dwordx4 %1, off offset:2048"
+				:: "s"(lds), "v"(g) : "memory", "m0");
+		break;
+	default:
+		asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off\n\t"
+				"global_load_lds_dwordx4 %1, off offset:1024\n\tglobal_load_lds_dwordx4 %1, off offset:2048\n\t"
+				"global_load_lds_dwordx4 %1, off offset:3072"
+				:: "s"(lds), "v"(g) : "memory", "m0");
+		break;
+	}
 }
 #pragma clang diagnostic pop
 
@@ -243,15 +376,40 @@ __device__ __forceinline__ void wait_vmcnt_at_most(uint32_t n)
 	else MISTA_VMCNT(0);
 }
 
+// Four LDS reads of two consecutive entries each (byte addresses addr),
+// issued back to back and waited for once, in one asm block: the compiler
+// would otherwise wait for each read before the next.  The outputs are
+// early-clobbered so no address register is reused while a read is in flight.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+template<uint32_t ESZ>
+__device__ __forceinline__ void lds_read_pairs4(const uint32_t* addr, typename LdsPair<ESZ>::T* out)
+{
+	if constexpr (ESZ == 8) {
+		asm volatile("ds_read2_b64 %0, %4 offset1:1\n\tds_read2_b64 %1, %5 offset1:1\n\t"
+				"ds_read2_b64 %2, %6 offset1:1\n\tds_read2_b64 %3, %7 offset1:1\n\ts_waitcnt lgkmcnt(0)"
+				: "=&v"(out[0]), "=&v"(out[1]), "=&v"(out[2]), "=&v"(out[3])
+				: "v"(addr[0]), "v"(addr[1]), "v"(addr[2]), "v"(addr[3]) : "memory");
+	} else {
+		asm volatile("ds_read2_b32 %0, %4 offset1:1\n\tds_read2_b32 %1, %5 offset1:1\n\t"
+				"ds_read2_b32 %2, %6 offset1:1\n\tds_read2_b32 %3, %7 offset1:1\n\ts_waitcnt lgkmcnt(0)"
+				: "=&v"(out[0]), "=&v"(out[1]), "=&v"(out[2]), "=&v"(out[3])
+				: "v"(addr[0]), "v"(addr[1]), "v"(addr[2]), "v"(addr[3]) : "memory");
+	}
+}
+#pragma clang diagnostic pop
+
 // Chunk i of a tile = pairs (rx r, tx c*TC .. c*TC+TC-1), r = i / CPR,
 // c = i % CPR.  Chunks stream through a ring of nbuf LDS buffers, each
-// holding TC windows of wcap entries plus the tile's rx delays (one float per
-// lane, copied with the first chunk of every rx).  Chunk i + nbuf - 1 is
-// copied while chunk i is computed, so nbuf - 1 chunks of copies are in
-// flight.  Every wave issues the same number of copies per chunk (windows
-// narrower than the widest re-copy their first slot into unused space), so
-// the number of copies issued after chunk i is known and the wave waits for
-// chunk i alone with s_waitcnt vmcnt.
+// holding TC windows of S 1 KiB slots plus the tile's rx delays (one float
+// per lane, copied with the first chunk of every rx).  Chunk i + nbuf - 1 is
+// copied while chunk i is computed.  Wave w copies the windows of pairs
+// w, w + NW, ... of each chunk, every one as S whole slots from its first
+// entry, so every wave issues the same number of copies per chunk and waits
+// for chunk i alone with s_waitcnt vmcnt.  The chunk's window words and
+// weights are scalar-loaded one chunk ahead; its TC LDS reads are issued
+// together and waited for once.  Chunks in which no pair can reach a record
+// end (the common case) take a path without the range test.
 template<int STORAGE, int CF, int NTXMAX, int TC, int NTHREADS, bool MULTI>
 __global__ void __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(3, 3)))
 das_tile_kernel(const TileArgs A)
@@ -259,10 +417,14 @@ das_tile_kernel(const TileArgs A)
 	using E = typename Entry<STORAGE>::T;
 	constexpr uint32_t NT = NTHREADS;       // one tile point per lane
 	constexpr uint32_t ESZ = sizeof(E);
-	constexpr uint32_t EPL = 16 / ESZ;      // entries per 16-byte lane load
-	constexpr uint32_t SLOT = 64 * EPL;     // entries per wave load instruction
+	constexpr uint32_t SLOT = 64 * 16 / ESZ;   // entries per wave load instruction (1 KiB)
 	constexpr uint32_t NW = NT / 64;
+	static_assert(TC % NW == 0, "every wave copies whole windows");
+	constexpr uint32_t PPW = TC / NW;       // windows copied per wave per chunk
 	constexpr uint32_t CPB = NTXMAX / TC;   // chunks per block of NTXMAX transmits
+	// LDS reads issued together: four where registers allow, one at a time
+	// when 128 registers hold transmit delays
+	constexpr int BATCH = NTXMAX <= 64 ? 4 : 1;
 	extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
 
 	const DasArgs& a = A.d;
@@ -282,19 +444,20 @@ das_tile_kernel(const TileArgs A)
 	// transmits run in blocks of NTXMAX (MULTI: several, at NTXMAX = 128)
 	const uint32_t nblk = MULTI ? A.ntx_pad / NTXMAX : 1u, CPR = nblk * CPB;   // chunks per rx
 	const float lim = static_cast<float>(L - 1u);
-	const uint32_t wcap = A.wcap, NB = A.nbuf;
-	const uint32_t S = wcap / SLOT;                   // slots of the widest window
-	const uint32_t QW = TC * S / NW;                  // window copies per wave per chunk
-	const uint32_t win_bytes = TC * wcap * ESZ;
+	const uint32_t S = A.S, NB = A.nbuf;
+	const uint32_t wcap = S * SLOT;
+	const uint32_t win_bytes = TC * S * 1024u;
 	const uint32_t buf_bytes = win_bytes + NT * 4;    // windows + rx delays
-	const cu2_t prm = (cu2_t)(A.params + static_cast<uint64_t>(tile) * nrx * A.ntx_pad);
+	const cu32_t wtab = (cu32_t)(A.words + static_cast<uint64_t>(tile) * nrx * A.ntx_pad);
 	const unsigned char* tensor = static_cast<const unsigned char*>(a.tensor);
+	const uint64_t chan_bytes = static_cast<uint64_t>(L) * ESZ;        // one channel
+	const uint64_t tx_bytes = chan_bytes * nrx;                        // one transmit's channels
 	const uint32_t lds0 = static_cast<uint32_t>(reinterpret_cast<size_t>((__attribute__((address_space(3))) void*)smem));
 
 	// tx delays of this lane's point for transmit block blk, offset folded
 	// in: (offset + d_tx).  Transmits are padded to whole blocks: padded ones
-	// have weight 0, an empty window and a position far below 0, so they
-	// take the excluded path.
+	// have weight 0, position far below 0 and copy transmit ntx - 1's
+	// window, so they take the excluded path.
 	float dtx[NTXMAX];
 	auto load_dtx = [&](uint32_t blk) {
 #pragma unroll
@@ -307,25 +470,47 @@ das_tile_kernel(const TileArgs A)
 	};
 	load_dtx(0);
 
-	// Issue the copies of chunk i into buffer b, given its window parameters
-	// pp.  Slot sl of pair j is copy number j * S + sl of the chunk and goes
-	// to wave (j * S + sl) % NW.
 	const uint32_t nchunks = nrx * CPR;
-	auto issue = [&](uint32_t i, uint32_t b, const WinParam* pp) {
-		const uint32_t r = i / CPR, c = i % CPR;   // c == 0: first chunk of rx r
-		const uint32_t bbase = lds0 + b * buf_bytes;
+	const uint32_t Q = PPW * S;   // window copies per wave per chunk (+1 for the rx delays at an rx start)
+	const uint32_t S4 = S & ~3u, SR = S & 3u;   // slots in groups of four, and the rest
+	// Window words and weights are laid out chunk after chunk: chunk k's
+	// pairs are entries k * TC .. k * TC + TC - 1 of the tile's table (the
+	// transmit rows are padded to whole chunks), and of the weights.
+	// Issue cursor: the next chunk to copy, its position in its rx,
+	// its ring buffer and the address of its first window's channel.
+	uint32_t is_k = 0, is_c = 0, is_lds = lds0;
+	const unsigned char* is_rbase = tensor;   // tensor + (rx of is_k) * chan_bytes
+	const unsigned char* is_cbase = tensor;   // + is_c * TC * tx_bytes
+	const float* is_drx = a.delay + p;        // rx delays of is_k's rx for this lane
+	uint32_t iw[PPW];                         // words of is_k's pairs that this wave copies
+	auto load_issue_words = [&]() {
+		const cu32_t pr = wtab + is_k * TC + wave;
 #pragma unroll
-		for (uint32_t j = 0; j < TC; ++j) {
-			const uint32_t slots = (pp[j].word >> 21) & 511u;
-			const unsigned char* src = tensor + static_cast<uint64_t>(pp[j].start) * ESZ + lane * 16;
-			const uint32_t dst = bbase + j * wcap * ESZ;
-			for (uint32_t sl = (wave - j * S) & (NW - 1); sl < S; sl += NW) {
-				const uint32_t from = sl < slots ? sl : 0u;
-				lds_dma16(src + from * SLOT * ESZ, __builtin_amdgcn_readfirstlane(dst + sl * SLOT * ESZ));
-			}
+		for (uint32_t q = 0; q < PPW; ++q) iw[q] = pr[q * NW];
+	};
+	auto issue = [&]() {
+#pragma unroll
+		for (uint32_t k = 0; k < PPW; ++k) {
+			const uint32_t j = wave + k * NW;
+			// padded transmits copy from the tensor's start (weight 0)
+			const unsigned char* src = (is_c * TC + j < ntx) ? is_cbase + j * tx_bytes + (iw[k] & ~kEdgeBit) : tensor;
+			src += lane * 16;
+			uint32_t dst = is_lds + j * S * 1024u;
+			for (uint32_t s0 = 0; s0 < S4; s0 += 4, src += 4096, dst += 4096) lds_dma16_slots(src, dst, 4);
+			if (SR) lds_dma16_slots(src, dst, SR);
 		}
-		if (c == 0)
-			lds_dma4(a.delay + static_cast<uint64_t>(r) * npix + p, __builtin_amdgcn_readfirstlane(bbase + win_bytes + wave * 256));
+		if (is_c == 0) lds_dma4(is_drx, __builtin_amdgcn_readfirstlane(is_lds + win_bytes + wave * 256));
+		// advance to the next chunk
+		++is_k;
+		is_lds += buf_bytes;
+		if (is_lds == lds0 + NB * buf_bytes) is_lds = lds0;
+		is_cbase += TC * tx_bytes;
+		if (++is_c == CPR) {
+			is_c = 0;
+			is_rbase += chan_bytes;
+			is_cbase = is_rbase;
+			is_drx += npix;
+		}
 	};
 	// Windows of pairs with nothing to stage are read (with weight 0): start
 	// from zeros so that no stale bit pattern can be Inf/NaN.
@@ -333,89 +518,133 @@ das_tile_kernel(const TileArgs A)
 		reinterpret_cast<uint4*>(smem)[i] = make_uint4(0u, 0u, 0u, 0u);
 	__syncthreads();
 
-	// Window parameters of the next chunk to issue, loaded one chunk ahead:
-	// their scalar loads miss the caches (each tile streams its own table),
-	// and loading them at the issue itself would add that latency to the
-	// copies' on every chunk.
-	WinParam nx[TC];
-	auto load_params = [&](uint32_t k) {
-		const cu2_t pr = prm + (k / CPR) * A.ntx_pad + (k % CPR) * TC;
-#pragma unroll
-		for (uint32_t j = 0; j < TC; ++j) {
-			nx[j].start = pr[j].start;
-			nx[j].word = pr[j].word;
-		}
-	};
 	const uint32_t pre = min(NB - 1, nchunks);
 	for (uint32_t k = 0; k < pre; ++k) {
-		load_params(k);
-		issue(k, k, nx);
+		load_issue_words();
+		issue();
 	}
-	if (pre < nchunks) load_params(pre);
+	if (is_k < nchunks) load_issue_words();
+	// window words and weights of the chunk to compute, one chunk ahead
+	uint32_t cw[TC];
+	float cwt[TC];
+	auto load_chunk = [&](uint32_t k) {
+		const cu32_t pr = wtab + k * TC;
+		const cf32_t wr = (cf32_t)(a.w + static_cast<uint64_t>(k) * TC);
+#pragma unroll
+		for (uint32_t j = 0; j < TC; ++j) {
+			cw[j] = pr[j];
+			cwt[j] = wr[j];
+		}
+	};
+	load_chunk(0);
 
-	PixelStats<CF> st;
-	uint32_t i = 0, b = 0;
+	TileStats<CF> st;
+	st.m = reinterpret_cast<double*>(smem + NB * buf_bytes + tid * 32);
+	uint32_t i = 0, bbase = 0;   // chunk being computed, its buffer's LDS offset
 	for (uint32_t r = 0; r < nrx; ++r) {
 		float drx = 0.0f, ar = 0.0f, ai = 0.0f;
 		for (uint32_t blk = 0; blk < nblk; ++blk) {
 			if (MULTI) load_dtx(blk);   // waited for at its first use
 #pragma clang loop unroll(full)
 			for (int c = 0; c < static_cast<int>(CPB); ++c, ++i) {
+				// this chunk's words and weights (loaded one chunk ahead)
+				uint32_t w_[TC];
+				float wt[TC];
+#pragma unroll
+				for (int j = 0; j < TC; ++j) {
+					w_[j] = cw[j];
+					wt[j] = cwt[j];
+				}
 				// copies issued after chunk i: chunks i+1 .. min(i+NB-2, last),
-				// QW each plus one for every chunk that starts an rx
-				const uint32_t hi = min(i + NB - 2, nchunks - 1);
-				const uint32_t after = (hi > i) ? (hi - i) * QW + (hi / CPR - i / CPR) : 0u;
+				// Q each plus one for every chunk that starts an rx
+				const uint32_t cin = blk * CPB + static_cast<uint32_t>(c);   // chunk in the rx
+				uint32_t after;
+				if (i + NB - 2 < nchunks) after = (NB - 2) * Q + (cin + NB - 2) / CPR;
+				else after = (nchunks - 1 - i) * Q + (nrx - 1 - r);
 				wait_vmcnt_at_most(after);         // this wave's copies of chunk i landed
 				__syncthreads();                   // everyone's did; buffer of chunk i-1 is free
-				if (i + NB - 1 < nchunks) issue(i + NB - 1, (b + NB - 1) % NB, nx);
-				const uint32_t bbase = b * buf_bytes;
+				if (is_k < nchunks) issue();
 				if (c == 0 && blk == 0) drx = *reinterpret_cast<const float*>(smem + bbase + win_bytes + tid * 4);
 				// Register fences: this chunk's positions are computed after
 				// the barrier, and its accumulation is finished before the
 				// next one, so only one chunk's temporaries are ever live.
 				asm volatile("" : "+v"(drx));
-				const uint32_t t0 = blk * NTXMAX + c * TC;   // first transmit of the chunk
-				const cu2_t pr = prm + r * A.ntx_pad + t0;
-				const cf32_t wr = (cf32_t)(a.w + static_cast<uint64_t>(r) * a.w_stride + t0);
+				uint32_t any_edge = 0;
 #pragma unroll
-				for (int j = 0; j < TC; ++j) {
-					// One form for every pair: lanes outside 0 <= pos < L-1 get
-					// weight 0; the entry index floor(pos) - ws is clamped,
-					// as an unsigned value, to the window, which changes
-					// nothing for in-range lanes (it lies in [0, need - 2] by
-					// the window construction) and keeps excluded lanes'
-					// reads on staged (finite) entries, whose products with
-					// weight 0 leave the sums as they are (a sum is never -0).
-					const uint32_t ws = pr[j].word & kWsMask;
+				for (int j = 0; j < TC; ++j) any_edge |= w_[j];
+				const uint32_t wbase = lds0 + bbase;
+				// entry address and weight of pair j for this lane
+				auto fast_pair = [&](int j, uint32_t& addr, float& kf, float& wv) {
+					// every lane of every pair of the chunk is inside 0 <= pos < L - 1
 					const float pos = dtx[c * TC + j] + drx;
-					const float k = __builtin_amdgcn_fractf(pos);
-					// 0 <= pos < lim as one unsigned compare of the bit
-					// patterns: positions are never -0 (delays are >= +0), so
-					// non-negative ones order like their bits, and negative
-					// values and NaN compare above lim's
+					kf = __builtin_amdgcn_fractf(pos);
+					addr = static_cast<uint32_t>(pos) * ESZ + (wbase + static_cast<uint32_t>(j) * S * 1024u - w_[j]);
+					wv = wt[j];
+				};
+				auto edge_pair = [&](int j, uint32_t& addr, float& kf, float& wv) {
+					// One form for every pair: lanes outside 0 <= pos < L-1
+					// get weight 0; the entry index floor(pos) - ws is
+					// clamped, as an unsigned value, to the window, which
+					// changes nothing for in-range lanes (it lies in
+					// [0, need - 2] by the window construction) and keeps
+					// excluded lanes' reads on staged (finite) entries, whose
+					// products with weight 0 leave the sums as they are (a sum
+					// is never -0).  0 <= pos < lim is one unsigned compare of
+					// the bit patterns: positions are never -0 (delays are
+					// >= +0), so non-negative ones order like their bits, and
+					// negative values and NaN compare above lim's.  (The
+					// fenced copy of drx keeps the compiler from computing
+					// both paths' positions ahead of the branch.)
+					float dr = drx;
+					asm volatile("" : "+v"(dr));
+					const float pos = dtx[c * TC + j] + dr;
+					kf = __builtin_amdgcn_fractf(pos);
+					const uint32_t ws = (w_[j] & ~kEdgeBit) / ESZ;
 					const bool ok = __float_as_uint(pos) < __float_as_uint(lim);
-					// load the weight unconditionally (a select, not a branch
-					// around a scalar load)
-					float wv = wr[j];
-					asm volatile("" : "+s"(wv));
-					const float w = ok ? wv : 0.0f;
 					const float pc = __builtin_amdgcn_fmed3f(pos, 0.0f, lim);   // NaN-free, no int overflow
 					const uint32_t rel = min(static_cast<uint32_t>(static_cast<int>(pc)) - ws, wcap - 2u);
-					uint32_t sb = bbase + j * wcap * ESZ;   // uniform: one shift-add per lane
-					asm("" : "+s"(sb));
-					const E* e = reinterpret_cast<const E*>(smem + (rel * ESZ + sb));
-					const float2 v = Entry<STORAGE>::lerp(e, k);
-					ar = fmaf(v.x, w, ar);
-					ai = fmaf(v.y, w, ai);
-				}
+					addr = rel * ESZ + wbase + static_cast<uint32_t>(j) * S * 1024u;
+					wv = ok ? wt[j] : 0.0f;
+				};
+				auto accumulate = [&](const float2 v, float wv) {
+					ar = fmaf(v.x, wv, ar);
+					ai = fmaf(v.y, wv, ai);
+				};
+				auto run = [&](auto&& pair) {
+					if constexpr (BATCH == 4) {
+						// groups of four pairs: four LDS reads in flight
+#pragma unroll
+						for (int g4 = 0; g4 < TC; g4 += 4) {
+							uint32_t addr[4];
+							float kf[4], wv[4];
+#pragma unroll
+							for (int q = 0; q < 4; ++q) pair(g4 + q, addr[q], kf[q], wv[q]);
+							typename LdsPair<ESZ>::T e[4];
+							lds_read_pairs4<ESZ>(addr, e);
+#pragma unroll
+							for (int q = 0; q < 4; ++q) accumulate(Entry<STORAGE>::lerp2(e[q], kf[q]), wv[q]);
+						}
+					} else {
+#pragma unroll
+						for (int j = 0; j < TC; ++j) {
+							uint32_t addr;
+							float kf, wv;
+							pair(j, addr, kf, wv);
+							const E* e = reinterpret_cast<const E*>(smem + (addr - lds0));
+							accumulate(Entry<STORAGE>::lerp(e, kf), wv);
+						}
+					}
+				};
+				if (!(any_edge & kEdgeBit)) run(fast_pair);
+				else run(edge_pair);
 				asm volatile("" : "+v"(ar), "+v"(ai));
-				// after this chunk's LDS reads (a pending scalar load would
-				// make every LDS wait a full drain); the chunk index passes
-				// through a fence so the loads stay here
-				uint32_t nk = i + NB;
-				asm volatile("" : "+s"(nk));
-				if (nk < nchunks) load_params(nk);
-				b = (b + 1 == NB) ? 0u : b + 1;
+				// the next chunks' scalars, after this chunk's LDS reads (a
+				// scalar load in flight would hold up their wait): in flight
+				// during the next barrier
+				if (is_k < nchunks) load_issue_words();
+				if (i + 1 < nchunks) load_chunk(i + 1);
+				bbase += buf_bytes;
+				if (bbase == NB * buf_bytes) bbase = 0;
 			}
 		}
 		st.add_rx(ar, ai, r == 0);
@@ -424,18 +653,17 @@ das_tile_kernel(const TileArgs A)
 	if (valid) st.write(a, p);
 }
 
-// LDS bytes of one chunk buffer: TC windows of wcap entries + the rx delays.
-template<int STORAGE>
-inline size_t tile_buffer_bytes(uint32_t tc, uint32_t wcap, uint32_t nt)
+// LDS bytes of one chunk buffer: TC windows of S 1 KiB slots + the rx delays.
+inline size_t tile_buffer_bytes(uint32_t tc, uint32_t S, uint32_t nt)
 {
-	return static_cast<size_t>(tc) * wcap * sizeof(typename Entry<STORAGE>::T) + nt * 4;
+	return static_cast<size_t>(tc) * S * 1024u + nt * 4;
 }
 
 template<int STORAGE, int CF, int NTXMAX>
 void launch_tile_t(const TileArgs& A, uint32_t ntiles, uint32_t tc, hipStream_t s)
 {
 	const uint32_t nt = A.tg.tw1 * A.tg.tw2 * A.tg.tzh;   // 256 or 512 (512: tc = 8)
-	const size_t lds = A.nbuf * tile_buffer_bytes<STORAGE>(tc, A.wcap, nt);
+	const size_t lds = A.nbuf * tile_buffer_bytes(tc, A.S, nt) + tile_stats_bytes(CF, nt);
 	auto go = [&](auto kern, uint32_t threads) {
 		HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
 				static_cast<int>(lds)));

@@ -6,6 +6,8 @@
 // Gaussian noise from a counter-based generator is added.
 #include "common.h"
 
+#include <algorithm>
+
 #include <cmath>
 
 namespace mista {
@@ -140,6 +142,58 @@ double sum_squares(const float* d_x, uint64_t n, hipStream_t s)
 	double t = 0.0;
 	for (double v : h) t += v;
 	return t;
+}
+
+// Streaming read of n 16-byte words: every lane reads 8 words per
+// iteration (128 B in flight per lane), a grid of 8 workgroups per CU
+// strides over the buffer, and one folded word per workgroup is written so
+// that nothing is optimised away.  Measures the achievable HBM read rate.
+__global__ void __launch_bounds__(256) read_stream_kernel(const uint4* __restrict__ x, uint64_t n, uint32_t* __restrict__ out)
+{
+	constexpr uint32_t U = 8;
+	uint32_t acc = 0;
+	const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+	uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	for (; i + (U - 1) * stride < n; i += U * stride) {
+		uint4 v[U];
+#pragma unroll
+		for (uint32_t k = 0; k < U; ++k) v[k] = x[i + k * stride];
+#pragma unroll
+		for (uint32_t k = 0; k < U; ++k) acc ^= v[k].x ^ v[k].y ^ v[k].z ^ v[k].w;
+	}
+	for (; i < n; i += stride) {
+		const uint4 v = x[i];
+		acc ^= v.x ^ v.y ^ v.z ^ v.w;
+	}
+	for (int o = 32; o > 0; o >>= 1) acc ^= static_cast<uint32_t>(__shfl_xor(static_cast<int>(acc), o, 64));
+	if (threadIdx.x == 0) out[blockIdx.x] = acc;
+}
+
+double read_bandwidth(const void* d_x, uint64_t bytes, int reps, hipStream_t s)
+{
+	const uint64_t n = bytes / 16;
+	if (n == 0 || reps < 1) MISTA_THROW(MISTA_E_INVALID_PARAM, "read_bandwidth: empty buffer or no repetitions.");
+	int dev = 0, cus = 0;
+	HIP_CHECK(hipGetDevice(&dev));
+	HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+	const uint32_t blocks = static_cast<uint32_t>(std::max(1, cus) * 8);
+	DeviceBuffer out;
+	uint32_t* d_out = static_cast<uint32_t*>(out.reserve(blocks * sizeof(uint32_t)));
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	HIP_CHECK(hipEventCreate(&e0));
+	HIP_CHECK(hipEventCreate(&e1));
+	hipLaunchKernelGGL(read_stream_kernel, dim3(blocks), dim3(256), 0, s, static_cast<const uint4*>(d_x), n, d_out);   // warm-up
+	HIP_CHECK(hipEventRecord(e0, s));
+	for (int r = 0; r < reps; ++r)
+		hipLaunchKernelGGL(read_stream_kernel, dim3(blocks), dim3(256), 0, s, static_cast<const uint4*>(d_x), n, d_out);
+	HIP_CHECK(hipEventRecord(e1, s));
+	HIP_CHECK(hipGetLastError());
+	HIP_CHECK(hipEventSynchronize(e1));
+	float ms = 0.0f;
+	HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	return static_cast<double>(n) * 16.0 * reps / (ms * 1e-3) / 1e9;
 }
 
 void add_noise(float* d_x, uint64_t n, float sigma, uint64_t seed, hipStream_t s)

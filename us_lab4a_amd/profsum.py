@@ -1,11 +1,16 @@
 """Turn rocprofv3 CSV output into the short text summaries kept in profiles/.
 
     python -m us_lab4a_amd.profsum trace KERNEL_STATS_CSV OUT.md --title T --frames F --command CMD
+    python -m us_lab4a_amd.profsum sq OUT.md DIR [DIR ...] --kernel SUBSTR --units N --title T --command CMD
     python -m us_lab4a_amd.profsum traffic FETCH_DIR WRITE_DIR OUT.md --kernel SUBSTR
             --calib SUBSTR --calib-bytes N --title T --command CMD
 
 trace:   per-kernel calls / average / total / per-frame time from
          rocprofv3 --kernel-trace --stats (<prefix>_kernel_stats.csv).
+sq:      instruction mix of the kernel matching --kernel from rocprofv3 --pmc
+         runs of SQ counters (any number of directories), per 64 of
+         --units (pixel-pairs per dispatch); writes a line "per_wave_pair:
+         valu=.. salu=.. lds=.. lds_conflict=.." that bench.py reads.
 traffic: HBM-side bytes of one dispatch of the kernel matching --kernel from
          two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch),
          each scaled by what the counter reads for a calibration kernel
@@ -83,6 +88,40 @@ def traffic(args):
     open(args.out, "w").write("\n".join(out) + "\n")
 
 
+def sq(args):
+    vals = {}
+    name = None
+    for d in args.dirs:
+        files = glob.glob(os.path.join(d, "*counter_collection.csv"))
+        if not files:
+            sys.exit(f"no counter_collection.csv under {d}")
+        per = {}
+        for r in csv.DictReader(open(files[0])):
+            if args.kernel not in r["Kernel_Name"]:
+                continue
+            name = r["Kernel_Name"]
+            per.setdefault(r["Counter_Name"], {}).setdefault(r["Dispatch_Id"], 0.0)
+            per[r["Counter_Name"]][r["Dispatch_Id"]] += float(r["Counter_Value"])
+        for c, disp in per.items():
+            vals[c] = sum(disp.values()) / len(disp)
+    if not vals:
+        sys.exit(f"no kernel matching {args.kernel!r}")
+    wp = args.units / 64.0
+    out = [f"# {args.title}", "", "Commands (repo root, MI355X; one counter group per run; per-dispatch means):", ""]
+    out += [f"    {c}" for c in args.command]
+    out += ["", f"Kernel: `{_clean(name)}`", "", f"Per wave-pair = per 64 of {args.units:.4g} units per dispatch.", "",
+            "| counter | per dispatch | per wave-pair |", "|---|---|---|"]
+    for c in sorted(vals):
+        out.append(f"| {c} | {vals[c]:.4g} | {vals[c] / wp:.4g} |")
+    key = {"valu": "SQ_INSTS_VALU", "salu": "SQ_INSTS_SALU", "lds": "SQ_INSTS_LDS", "lds_conflict": "SQ_LDS_BANK_CONFLICT",
+           "smem": "SQ_INSTS_SMEM", "vmem": "SQ_INSTS_VMEM", "wait_any": "SQ_WAIT_ANY", "wave_cycles": "SQ_WAVE_CYCLES"}
+    kv = " ".join(f"{k}={vals[c] / wp:.4g}" for k, c in key.items() if c in vals)
+    if args.note:
+        out += ["", args.note]
+    out += ["", f"per_wave_pair: {kv}"]
+    open(args.out, "w").write("\n".join(out) + "\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -103,8 +142,16 @@ def main():
     r.add_argument("--algorithmic", type=int, default=0)
     r.add_argument("--title", required=True)
     r.add_argument("--command", action="append", required=True)
+    q = sub.add_parser("sq")
+    q.add_argument("out")
+    q.add_argument("dirs", nargs="+")
+    q.add_argument("--kernel", required=True)
+    q.add_argument("--units", type=float, required=True)
+    q.add_argument("--title", required=True)
+    q.add_argument("--command", action="append", required=True)
+    q.add_argument("--note", default="")
     args = ap.parse_args()
-    (trace if args.cmd == "trace" else traffic)(args)
+    {"trace": trace, "traffic": traffic, "sq": sq}[args.cmd](args)
 
 
 if __name__ == "__main__":
