@@ -185,6 +185,21 @@ int mista_prepare_signals_device(mista_handle* h, const float* d_echoes, uint32_
 /* Delay-and-sum only over the handle's resident tensor. */
 int mista_das_device(mista_handle* h, float* d_value, float* d_factor, void* stream);
 
+/* No counterpart in the reference: mista_das_device for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same inputs and outputs and the same
+ * images as mista_das_device with the tiled kernel; it calls no hipMalloc
+ * or hipFree.  The handle must hold a prepared frame (the preparation sizes
+ * every buffer the DAS reads).  The window tables of the launch live in a
+ * static device array of 80 MiB that each GPU holds once per process and
+ * that every call on that GPU rebuilds: a geometry whose tables need more
+ * is refused with MISTA_E_INVALID_PARAM, and calls that overlap on one GPU
+ * (from several threads or streams) race.  Uses the tile shape tuned for the
+ * geometry when there is one, else a default; it never tunes.  Stage timing
+ * (mista_last_timing) is recorded only if an earlier frame created the
+ * handle's events.  Synchronises the stream once (to size the windows). */
+int mista_das_device_noalloc(mista_handle* h, float* d_value, float* d_factor, void* stream);
+
 /* Test access to the resident analytic tensor: host complex float32
  * [ntx][nrx][L][2].  Upload converts to the storage type (with per-channel
  * scale for F16/BF16); download converts back to float32. */

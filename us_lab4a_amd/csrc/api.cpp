@@ -206,6 +206,9 @@ void set_frame_geom(mista_handle* h, uint32_t ns, uint32_t L, hipStream_t s)
 	if (!h->capturing) HIP_CHECK(hipMemsetAsync(t + tbytes, 0, tensor_tail_padding(), s));
 	h->d_scale.reserve(nch * sizeof(float));
 	h->d_w.reserve(static_cast<size_t>(g.ntx_pad) * g.nrx * sizeof(float));
+	// apodization buffers too, so that the DAS of a prepared frame allocates nothing
+	h->d_rx_apod.reserve(static_cast<size_t>(g.nrx) * sizeof(float));
+	h->d_tx_apod.reserve(static_cast<size_t>(g.ntx) * sizeof(float));
 }
 
 void ensure_filter(mista_handle* h, hipStream_t s)
@@ -750,6 +753,40 @@ int mista_das_device(mista_handle* h, float* d_value, float* d_factor, void* str
 	return guarded(h, [&] {
 		if (!d_value) MISTA_THROW(MISTA_E_INVALID_PARAM, "null device pointer.");
 		run_das(h, d_value, d_factor, pick_stream(h, stream));
+	});
+}
+
+int mista_das_device_noalloc(mista_handle* h, float* d_value, float* d_factor, void* stream)
+{
+	if (!h) return MISTA_E_INVALID_PARAM;
+	return guarded(h, [&] {
+		if (!d_value) MISTA_THROW(MISTA_E_INVALID_PARAM, "null device pointer.");
+		require_grid(h);
+		if (!h->signal_ready) MISTA_THROW(MISTA_E_INVALID_STATE, "No prepared signals: call a prepare/process/upload entry point first.");
+		if (h->tuning.variant == 1) MISTA_THROW(MISTA_E_INVALID_STATE, "The no-allocation DAS runs the tiled kernel only.");
+		hipStream_t s = pick_stream(h, stream);
+		h->geom.n1 = h->n1;
+		h->geom.n2 = h->n2;
+		h->geom.ny = h->ny;
+		h->geom.npix = static_cast<uint64_t>(h->n1) * h->n2;
+		h->geom.grid_version = h->grid_version;
+		upload_apod(h, s);   // into buffers the preparation reserved
+		launch_weights(h->geom, h->d_rx_apod.as<const float>(), h->tx_apod_set ? h->d_tx_apod.as<const float>() : nullptr,
+				h->d_scale.as<const float>(), h->d_w.as<float>(), s);
+		const bool timed = h->ev[2] && h->ev[3];   // events are only recorded when they exist already
+		if (timed) HIP_CHECK(hipEventRecord(h->ev[2], s));
+		int p0 = h->das_scratch.tuned_p0, p1 = h->das_scratch.tuned_p1;
+		if (h->tuning.variant == 2 || h->tuning.p0 || h->tuning.p1) {
+			p0 = h->tuning.p0;
+			p1 = h->tuning.p1;
+		}
+		launch_das_noalloc(h->geom, p0, p1, h->das_scratch, h->d_tensor.get(), h->d_delay.as<const float>(),
+				h->d_w.as<const float>(), d_value, d_factor, s);
+		if (timed) {
+			HIP_CHECK(hipEventRecord(h->ev[3], s));
+			h->timed = true;
+			h->graph_frame = false;
+		}
 	});
 }
 

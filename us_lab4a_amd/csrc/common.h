@@ -186,7 +186,12 @@ struct DasScratch {
 };
 // Bytes of zeroed padding after the analytic tensor (window over-read).
 size_t tensor_tail_padding();
-
+// Device scratch of the no-allocation DAS entry point (per GPU, static).
+constexpr size_t kNoAllocScratchBytes = static_cast<size_t>(80) << 20;
+// DAS over the handle's resident tensor with its window tables in that
+// scratch: no device allocation; the tuned tile shape when there is one.
+void launch_das_noalloc(const FrameGeom& g, int tuned_p0, int tuned_p1, DasScratch& last, const void* d_tensor,
+		const float* d_delay, const float* d_w, float* d_value, float* d_factor, hipStream_t s);
 void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, const void* d_tensor,
 		const float* d_delay, const float* d_w, float* d_value, float* d_factor, hipStream_t s);
 

@@ -23,6 +23,8 @@
 #include "das_common.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <vector>
 
@@ -388,6 +390,67 @@ DasArgs das_args(const FrameGeom& g, const void* d_tensor, const float* d_delay,
 	a.inv_num = 1.0f / static_cast<float>(static_cast<uint64_t>(g.ntx) * g.nrx);
 	a.signed_value = (g.storage == kStorageReal) ? 1 : 0;
 	return a;
+}
+
+// ---------------------------------------------------------------- no allocation
+
+// Device scratch of the no-allocation DAS: the per-tile delay bounds, the
+// window words and the longest-window counter of one launch.  Every call on
+// a GPU shares that GPU's copy (each GPU the library runs on holds one until
+// the process ends) and rebuilds what it needs: calls that overlap on one GPU
+// race.  In host code `scratch` names a host-side stand-in; kernels get the
+// device address from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_ptr;   // device ordinal -> that GPU's copy of scratch
+}  // namespace
+
+// The current device's copy of scratch (looked up once per device, with the
+// device current).
+static unsigned char* noalloc_scratch()
+{
+	int dev = 0;
+	HIP_CHECK(hipGetDevice(&dev));
+	std::lock_guard<std::mutex> g(g_scratch_mu);
+	auto it = g_scratch_ptr.find(dev);
+	if (it != g_scratch_ptr.end()) return it->second;
+	void* ptr = nullptr;
+	HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+	g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+	return static_cast<unsigned char*>(ptr);
+}
+
+void launch_das_noalloc(const FrameGeom& g, int tuned_p0, int tuned_p1, DasScratch& last, const void* d_tensor,
+		const float* d_delay, const float* d_w, float* d_value, float* d_factor, hipStream_t s)
+{
+	DasArgs a = das_args(g, d_tensor, d_delay, d_w, d_value, d_factor);
+	if (!((g.ntx_pad <= 128 || g.ntx_pad % 128 == 0) && static_cast<uint64_t>(g.L) * storage_bytes(g.storage) < (1ull << 31)))
+		MISTA_THROW(MISTA_E_INVALID_PARAM, "This geometry has no tiled DAS form, which the no-allocation entry point needs.");
+	unsigned char* base = noalloc_scratch();
+	// the tuned tile shape when it fits, else the default, else the larger tiles
+	std::vector<std::pair<int, int>> shapes;
+	if (tuned_p0 >= 0) shapes.emplace_back(tuned_p0, tuned_p1);
+	for (int p0 : (g.ny > 1 ? std::vector<int>{4, 7} : std::vector<int>{1, 5, 6})) shapes.emplace_back(p0, 0);
+	size_t need_min = ~size_t(0);
+	for (const auto& sh : shapes) {
+		const TilePlan tp = tile_plan(g, sh.first);
+		if (!tp.ok) continue;
+		const size_t bb = (bounds_bytes(g, tp) + 255) / 256 * 256, wb = words_bytes(g, tp);
+		const size_t need = bb + wb + 256;
+		need_min = std::min(need_min, need);
+		if (need > kNoAllocScratchBytes) continue;
+		float2* bounds = reinterpret_cast<float2*>(base);
+		uint32_t* words = reinterpret_cast<uint32_t*>(base + bb);
+		uint32_t* d_max = reinterpret_cast<uint32_t*>(base + bb + wb);
+		const uint32_t S = build_window_tables(g, tp, d_delay, bounds, words, d_max, s);
+		if (launch_tiles(g, tp, sh.first, sh.second, S, words, a, last, s)) return;
+	}
+	if (need_min != ~size_t(0) && need_min > kNoAllocScratchBytes)
+		MISTA_THROW(MISTA_E_INVALID_PARAM, "The DAS window tables of this geometry need " << need_min
+				<< " bytes; the no-allocation entry point holds at most " << kNoAllocScratchBytes << '.');
+	MISTA_THROW(MISTA_E_INVALID_PARAM, "The signal windows of this geometry do not fit the tiled DAS kernel.");
 }
 
 // Tail padding of the analytic tensor (bytes): a staged window is copied as

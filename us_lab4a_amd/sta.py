@@ -226,6 +226,11 @@ class HipSTAProcessor:
     def das_device(self, d_value, d_factor=None, stream=None):
         check(lib().mista_das_device(self._h, _ptr(d_value), _ptr(d_factor), _stream_ptr(stream)), self._h)
 
+    def das_device_noalloc(self, d_value, d_factor=None, stream=None):
+        """das_device without device allocation (window tables in the
+        library's static scratch; see mista_das_device_noalloc)."""
+        check(lib().mista_das_device_noalloc(self._h, _ptr(d_value), _ptr(d_factor), _stream_ptr(stream)), self._h)
+
     def upload_analytic(self, analytic):
         """analytic: complex [ntx][nrx][L] (complex64/128) or float32 [..][L][2]."""
         a = np.asarray(analytic)
