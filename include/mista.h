@@ -266,9 +266,11 @@ int mista_envelope_device(mista_handle* h, float* d_data, uint32_t n1, uint32_t 
 int mista_envelope(mista_handle* h, float* data, uint32_t n1, uint32_t n2);
 
 /* Select the signal-preparation implementation: 0 = automatic (fused LDS
- * kernel when the FFT fits on chip, else the library FFT), 1 = library FFT
+ * kernel when the FFT fits on chip, else the four-step FFTs), 1 = library FFT
  * (FIR kernel, hipFFT R2C, spectrum kernel, hipFFT C2C, store kernel),
- * 2 = fused LDS kernel (error if the frame does not fit). */
+ * 2 = fused LDS kernel (error if the frame does not fit), 3 = four-step FFTs
+ * (FIR kernel, two-pass forward FFT, spectrum kernel, two-pass backward FFT,
+ * store kernel; FFT sizes up to 2^20 whose two factors are 2-3-5 smooth). */
 int mista_set_prep(mista_handle* h, int method);
 
 /* Stage times of the last mista_das_device/mista_process_device frame (ms,
@@ -278,13 +280,19 @@ int mista_last_timing(const mista_handle* h, float* prep_ms, float* das_ms);
 
 /* ---- serving engine (no reference counterpart) ----
  * A request loop for imaging jobs of uneven shapes on one GPU.  Jobs are
- * admitted in arrival order as in-flight slots and device memory free up;
- * each runs on a resident processor handle chosen by (configuration, grid,
- * sample count), so delay tables, window tables and FFT plans stay in HBM
- * between jobs of the same shape.  Admitted jobs run on separate streams and
- * overlap on the device (continuous batching).  A job's images do not depend
- * on the other jobs in flight: they are bit-identical to the same job run
- * alone.  submit/wait/query may be called from any thread.
+ * admitted as in-flight slots and device memory free up, in arrival order
+ * except that a job that can start now may pass one that must wait (the
+ * oldest job is passed at most 32 times).  Each runs on a resident processor
+ * handle chosen by (configuration, apodization, grid, sample count), so delay
+ * tables, window tables, the tuned tile shape and FFT plans stay in HBM
+ * between jobs of the same shape; a shape holds up to handles_per_shape
+ * handles with two jobs queued on each handle's stream, so one frame's
+ * preparation overlaps the previous frame's DAS.  Handle creation, table
+ * builds and tuning run on a builder thread, never on the admitting thread.
+ * Admitted jobs run on separate streams and overlap on the device
+ * (continuous batching).  A job's images do not depend on the other jobs in
+ * flight: they are bit-identical to the same job run alone.
+ * submit/wait/query may be called from any thread.
  */
 typedef struct mista_engine mista_engine;
 
@@ -292,7 +300,7 @@ typedef struct mista_engine_config {
 	int32_t device;
 	uint32_t max_inflight;       /* jobs on the device at once, 1..64          */
 	uint32_t max_queued;         /* jobs waiting for admission; more -> BUSY    */
-	uint32_t pad_;
+	uint32_t handles_per_shape;  /* resident handles per shape, 0..8 (0 = 2)    */
 	uint64_t hbm_budget;         /* device bytes the resident handles may hold
 	                                (0 = 80 % of the free memory at creation)  */
 } mista_engine_config;
@@ -319,7 +327,7 @@ typedef struct mista_job {
 
 typedef struct mista_job_stats {
 	int32_t status;              /* MISTA_OK or the job's error code            */
-	uint32_t pad_;
+	int32_t das_tile;            /* tile shape of its DAS launch (-1: gather)   */
 	double queued_ms;            /* submit -> admitted                          */
 	double latency_ms;           /* submit -> outputs ready (seen by the loop)  */
 	double device_ms;            /* job's work on its stream (HIP events)       */
@@ -336,6 +344,10 @@ int mista_engine_submit(mista_engine* e, const mista_job* job, uint64_t* ticket)
 int mista_engine_wait(mista_engine* e, uint64_t ticket, mista_job_stats* stats);
 int mista_engine_query(mista_engine* e, uint64_t ticket, int* done);
 int mista_engine_stats(mista_engine* e, uint64_t* resident_bytes, uint32_t* inflight, uint32_t* queued);
+/* Totals since creation: handles built (each builds its delay and window
+ * tables), autotuner runs, idle handles evicted; and handles being built now. */
+int mista_engine_counters(mista_engine* e, uint64_t* handles_created, uint64_t* tunes, uint64_t* evictions,
+		uint32_t* building);
 const char* mista_engine_last_error(const mista_engine* e);
 
 /* ---- synthetic echoes (test and benchmark input) ----

@@ -94,7 +94,7 @@ def run_das(p, wl, variant):
     return dv.cpu().numpy().reshape(wl.n_x, wl.n_z), df.cpu().numpy().reshape(wl.n_x, wl.n_z)
 
 
-@pytest.mark.parametrize("method", [1, 2], ids=["library_fft", "fused_lds"])
+@pytest.mark.parametrize("method", [1, 2, 3], ids=["library_fft", "fused_lds", "four_step"])
 @pytest.mark.parametrize("name", CASES)
 def test_edge_prepare_matches_fp64_oracle(name, method):
     wl, echoes, a64, _ = case_data(name)

@@ -22,7 +22,7 @@ HOST_DRIVER = os.path.join(PKG, "mista_sta_cli")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["api.cpp", "prep.hip", "das.hip", "das_tile_f32.hip", "das_tile_f16.hip", "das_tile_bf16.hip", "das_tile_rf.hip",
+SOURCES = ["api.cpp", "prep.hip", "fft4.hip", "das.hip", "das_tile_f32.hip", "das_tile_f16.hip", "das_tile_bf16.hip", "das_tile_rf.hip",
            "synth.hip", "serve.cpp"]
 
 HIP_FLAGS = [

@@ -980,7 +980,7 @@ int mista_set_prep(mista_handle* h, int method)
 	if (!h) return MISTA_E_INVALID_PARAM;
 	return guarded(h, [&] {
 		++h->settings_epoch;
-		if (method < 0 || method > 2) MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid preparation method " << method);
+		if (method < 0 || method > 3) MISTA_THROW(MISTA_E_INVALID_PARAM, "Invalid preparation method " << method);
 		h->prep.method = method;
 	});
 }

@@ -136,10 +136,17 @@ void launch_delay_table(const float* d_grid_xz, uint64_t npix, uint32_t num_elem
 void launch_delay_table_3d(const float* d_grid_xyz, uint64_t npix, const double* d_elem_xy, uint32_t rows,
 		double inv_ct, float* d_delay, hipStream_t s);
 
-struct PrepPlan;  // hipFFT plans + scratch for signal preparation
+struct PrepPlan;  // FFT plans + scratch for signal preparation
+// Batched four-step FFTs of long 2-3-5 smooth sizes (fft4.hip).
+struct Fft4Plan;
+Fft4Plan* fft4_plan(uint32_t n, hipStream_t s);
+void fft4_destroy(Fft4Plan* p);
+uint32_t fft4_size(const Fft4Plan* p);
+size_t fft4_device_bytes(const Fft4Plan* p);
+void fft4_exec(const Fft4Plan& p, const float2* in, float2* tmp, float2* out, uint32_t batch, hipStream_t s);
 struct PrepState {
 	PrepPlan* plan = nullptr;  // owned; destroyed in prep.hip where PrepPlan is complete
-	int method = 0;            // 0 automatic, 1 library FFT (hipFFT), 2 fused LDS kernel
+	int method = 0;            // 0 automatic (fused kernel, else four-step FFTs), 1 library FFT (hipFFT), 2 fused kernel, 3 four-step FFTs
 	bool last_fused = false;   // the last preparation ran the fused kernel
 	// called before the kernels of channels [c0, c0 + n) are launched (the
 	// host path uploads each chunk of echoes just in time)

@@ -250,19 +250,19 @@ __device__ inline uint32_t pack_f16x2(float re, float im)
 }
 
 // One workgroup per channel, everything between the raw echo and the
-// analytic signal on chip: FIR x U + DC removal into LDS, then the
-// analytic-signal FFTs of size N = Nf (HilbertEnvelope.h:142-251) as three
-// complex FFTs of size M = N/2 held in LDS:
+// analytic signal on chip: FIR x U + DC removal into LDS, then the analytic
+// signal of size N = Nf (HilbertEnvelope.h:142-251) with two complex FFTs of
+// size M = N/2 held in LDS:
 //   forward:  Z = DFT_M(x[2m] + i x[2m+1])            (real FFT of size N)
-//   spectrum: X[k] = E + W^k O, X[M-k] = conj(E - W^k O), W = exp(-2 pi i/N),
-//             E = (Z[k] + conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / (2i);
-//             bins 0 and M halved, bins above M dropped;
-//   inverse:  z[2m]   = (2/N) IDFT_M(Be)[m],  Be[k] = X[k], Be[0] = X[0]/2 + X[M]/2
-//             z[2m+1] = (2/N) IDFT_M(Bo)[m],  Bo[k] = X[k] conj(W^k), Bo[0] = X[0]/2 - X[M]/2
-// The inverses run as forward transforms of the conjugates.  FFTs are
+//   Hilbert:  the analytic signal is x + i y with y the Hilbert transform of
+//             x, Y[k] = -i sgn(k) X[k] (sgn 0 at bins 0 and N/2), a real
+//             signal: y[2m] + i y[2m+1] = IDFT_M(C)[m] / M with C[0] = 0,
+//             C[k] = i sin(2 pi k/N) Z[k] + cos(2 pi k/N) conj(Z[M-k]);
+//   output:   (x_i - mean, y_i), x recomputed from the echo and the filter,
+//             which stay in LDS after the transform buffer.
+// The inverse runs as a forward transform of the conjugate.  FFTs are
 // Stockham passes of radix 8/4/2/3/5: every thread reads its butterflies'
 // inputs into registers, a barrier, then writes the outputs in place.
-// Bo waits in a global scratch row while Be is transformed.
 
 constexpr int kFftThreads = 1024;
 constexpr uint32_t kFirJ = 27;   // taps per phase of the Kaiser upsampling design (every U)
@@ -283,7 +283,6 @@ struct AnalyticArgs {
 	const float* hpoly;    // [up][J]
 	const float2* tw;      // Stockham twiddles exp(-2 pi i * i * j / (Ns * R))
 	const float2* wpost;   // exp(-2 pi i k / N), k in [0, M/2]
-	float2* scratch;       // [blocks][M]
 	float2* out;           // [blocks][L] analytic signal, or
 	float* out_mag;        // [blocks][L] its magnitude (kOutMagnitude), or
 	uint32_t* out16;       // [blocks][L] FP16/BF16 pairs over a per-channel scale
@@ -417,11 +416,26 @@ __device__ void fft_forward(float2* D, const FftPlanArgs& P, const float2* __res
 	}
 }
 
+// FIR x U output i (no DC removal): the generic polyphase loop over the
+// echo and the filter in LDS, or the raw sample when U = 1.  Deterministic,
+// so a second evaluation gives the same bits as the first.
+__device__ __forceinline__ float fir_at(const AnalyticArgs& A, const float* xs, const float* hs, const float* src,
+		uint32_t i)
+{
+	if (A.up <= 1) return src[i];
+	const uint32_t m = i + A.off;
+	const float* hrow = hs + (m % A.up) * A.J;
+	const float* xrow = xs + A.J + m / A.up;
+	float acc = 0.0f;
+	for (uint32_t j = 0; j < A.J; ++j) acc = fmaf(hrow[j], xrow[-static_cast<int>(j)], acc);
+	return acc;
+}
+
 __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const AnalyticArgs A)
 {
 	extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
 	__shared__ double red[kFftThreads / 64];
-	__shared__ float redm[kFftThreads / 64];
+	__shared__ float redm[kFftThreads / 64], redx[kFftThreads / 64], redn[kFftThreads / 64];
 	const uint32_t tid = threadIdx.x, ch = blockIdx.x;
 	const uint32_t M = A.plan.M, N = A.N, L = A.L, up = A.up, J = A.J;
 	float2* D = reinterpret_cast<float2*>(lds);
@@ -442,6 +456,7 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 		__syncthreads();
 	}
 	double part = 0.0;
+	float xmax = -3.4e38f, xmin = 3.4e38f;   // extremes of the FIR output over [0, L) (storage scale)
 	if (up > 1 && J == kFirJ) {
 		// Lane per input position q: the window x[q-26..q] sits in registers
 		// and serves all U phases (outputs m = q*U + p, i = m - off); the
@@ -473,6 +488,8 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 						for (int j = 0; j < kFirJ; ++j) acc = fmaf(hrow[j], xw[j], acc);
 						o[k] = acc;
 						part += static_cast<double>(acc);
+						xmax = fmaxf(xmax, acc);
+						xmin = fminf(xmin, acc);
 					}
 					*reinterpret_cast<float4*>(Df + ibase + h) = make_float4(o[0], o[1], o[2], o[3]);
 				}
@@ -486,25 +503,21 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 				const int i = ibase + static_cast<int>(ph);
 				if (i >= 0 && i < static_cast<int>(L)) {
 					if (static_cast<uint32_t>(i) >= A.dz) part += static_cast<double>(acc);
+					xmax = fmaxf(xmax, acc);
+					xmin = fminf(xmin, acc);
 					Df[i] = acc;
 				}
 			}
 		}
 	} else for (uint32_t i = tid; i < L; i += kFftThreads) {
-		float acc;
-		if (up > 1) {
-			const uint32_t m = i + A.off;
-			const float* hrow = hs + (m % up) * J;
-			const float* xrow = xs + J + m / up;
-			acc = 0.0f;
-			for (uint32_t j = 0; j < J; ++j) acc = fmaf(hrow[j], xrow[-static_cast<int>(j)], acc);
-		} else {
-			acc = src[i];
-		}
+		const float acc = fir_at(A, xs, hs, src, i);
 		if (i >= A.dz) part += static_cast<double>(acc);
+		xmax = fmaxf(xmax, acc);
+		xmin = fminf(xmin, acc);
 		Df[i] = acc;
 	}
 	for (uint32_t i = L + tid; i < N; i += kFftThreads) Df[i] = 0.0f;
+	float mean = 0.0f;
 	if (A.flags & kRemoveDc) {
 		for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
 		if ((tid & 63) == 0) red[tid >> 6] = part;
@@ -515,150 +528,246 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 			red[0] = s;
 		}
 		__syncthreads();
-		const float mean = static_cast<float>(red[0] / static_cast<double>(L - A.dz));
+		mean = static_cast<float>(red[0] / static_cast<double>(L - A.dz));
 		for (uint32_t i = tid; i < L; i += kFftThreads) Df[i] = Df[i] - mean;
 	}
 	__syncthreads();
 
 	fft_forward(D, A.plan, A.tw);
 
-	// spectrum -> conj(Be) in LDS, conj(Bo) straight to the scratch row
-	float2* bo = A.scratch + static_cast<uint64_t>(ch) * M;
+	// Z = DFT_M(x[2m] + i x[2m+1]) -> conj(C), in place: C[0] = 0,
+	// C[k] = i sin(2 pi k/N) Z[k] + cos(2 pi k/N) conj(Z[M-k]), 0 < k < M
+	// (pairs k, M-k by one thread; C[M/2] = i Z[M/2]).
 	const uint32_t half = M / 2;
-	constexpr int kMaxPairs = static_cast<int>((kFftMaxM / 2 + kFftThreads) / kFftThreads);   // ceil((M/2+1)/threads)
-	float2 keep[kMaxPairs][2];
-#pragma unroll
-	for (int q = 0; q < kMaxPairs; ++q) {
-		const uint32_t k = tid + static_cast<uint32_t>(q) * kFftThreads;
-		if (k > half) continue;
-		const float2 a = D[k];
+	for (uint32_t k = tid; k <= half; k += kFftThreads) {
 		if (k == 0) {
-			keep[q][0] = make_float2(a.x, 0.0f);   // X[0]/2 + X[M]/2 = Re Z[0]
-			bo[0] = make_float2(a.y, 0.0f);        // X[0]/2 - X[M]/2 = Im Z[0]
+			D[0] = make_float2(0.0f, 0.0f);
+		} else if (2 * k == M) {
+			const float2 a = D[k];
+			D[k] = make_float2(-a.y, -a.x);   // conj(i Z)
 		} else {
-			const float2 b = D[M - k];
-			const float2 E = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
-			const float2 O = mul_mi(make_float2(0.5f * (a.x - b.x), 0.5f * (a.y + b.y)));
-			const float2 w = A.wpost[k];
-			const float2 t = cmul(w, O);
-			const float2 xk = cadd(E, t);
-			const float2 xm = conjf2(csub(E, t));
-			keep[q][0] = conjf2(xk);
-			keep[q][1] = conjf2(xm);
-			bo[k] = conjf2(cmul(xk, conjf2(w)));
-			if (k != M - k) bo[M - k] = conjf2(cmul(xm, make_float2(-w.x, -w.y)));
+			const float2 a = D[k], b = D[M - k];
+			const float2 w = A.wpost[k];      // (cos, -sin)
+			const float c = w.x, sn = -w.y;
+			const float2 ck = make_float2(fmaf(c, b.x, -sn * a.y), fmaf(sn, a.x, -c * b.y));
+			const float2 cm = make_float2(fmaf(-c, a.x, -sn * b.y), fmaf(sn, b.x, c * a.y));
+			D[k] = conjf2(ck);
+			D[M - k] = conjf2(cm);
 		}
 	}
 	__syncthreads();
-#pragma unroll
-	for (int q = 0; q < kMaxPairs; ++q) {
-		const uint32_t k = tid + static_cast<uint32_t>(q) * kFftThreads;
-		if (k > half) continue;
-		D[k] = keep[q][0];
-		if (k != 0 && k != M - k) D[M - k] = keep[q][1];
-	}
-	__syncthreads();
-
-	const float coef = 2.0f / static_cast<float>(N);
-	const bool mag = (A.flags & kOutMagnitude) != 0;
-	float2* dst = A.out + static_cast<uint64_t>(ch) * L;
-	float* dmag = A.out_mag + static_cast<uint64_t>(ch) * L;
-	// sample n of the analytic signal: (2/N) conj(transform output)
-	auto put = [&](uint32_t n, float2 v) {
-		const float re = v.x * coef, im = -v.y * coef;
-		if (mag) dmag[n] = sqrtf(re * re + im * im);
-		else dst[n] = make_float2(re, im);
-	};
 	fft_forward(D, A.plan, A.tw);
+
+	// y[2m] + i y[2m+1] = conj(D[m]) / M: the Hilbert transform of x, the
+	// imaginary part of the analytic signal, whose real part is x itself
+	// (recomputed from the echo and filter kept in LDS).
+	const float coef = 2.0f / static_cast<float>(N);
+	auto hil = [&](uint32_t i) -> float {
+		const float2 v = D[i >> 1];
+		return (i & 1u) ? -v.y * coef : v.x * coef;
+	};
+	const bool mag = (A.flags & kOutMagnitude) != 0;
+	float inv = 1.0f;
 	if (A.out16) {
 		// FP16/BF16 storage over the channel scale s = max(|re|, |im|) (1 if
-		// zero), stored value z/s.  The even samples wait, in FP32, in the
-		// scratch row, swapped per index with the odd spectrum as it comes
-		// into LDS (index m is read and written by one thread only), while
-		// the odd transform runs; then each output pair is written at once.
-		float mx = 0.0f;
-		for (uint32_t m = tid; m < M; m += kFftThreads) {
-			const float2 od = bo[m];
-			if (2 * m < L) {
-				const float2 v = D[m];
-				const float2 e = make_float2(v.x * coef, -v.y * coef);
-				mx = fmaxf(mx, fmaxf(fabsf(e.x), fabsf(e.y)));
-				bo[m] = e;
-			}
-			D[m] = od;
-		}
-		__syncthreads();
-		fft_forward(D, A.plan, A.tw);
-		for (uint32_t m = tid; 2 * m + 1 < L; m += kFftThreads) {
-			const float2 v = D[m];
-			mx = fmaxf(mx, fmaxf(fabsf(v.x * coef), fabsf(-v.y * coef)));
-		}
-		for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-		if ((tid & 63) == 0) redm[tid >> 6] = mx;
-		__syncthreads();
-		float gm = redm[0];
-		for (int w = 1; w < kFftThreads / 64; ++w) gm = fmaxf(gm, redm[w]);
-		const float sc = (gm > 0.0f) ? gm : 1.0f;
-		const float inv = 1.0f / sc;
-		if (tid == 0) A.scale[ch] = sc;
-		const bool bf = (A.flags & kOutBf16) != 0;
-		uint32_t* row = A.out16 + static_cast<uint64_t>(ch) * L;
-		const bool pairs8 = (L & 1u) == 0;   // rows start 8-byte aligned
+		// zero), stored value z/s.  max |x_i - mean| is reached at the
+		// extremes of x (rounding is monotonic and odd): the workgroup's
+		// max x, max -x and max |y| are reduced first.
+		float my = 0.0f, nxmin = -xmin;
 		for (uint32_t m = tid; 2 * m < L; m += kFftThreads) {
-			const float2 e = bo[m];
-			const uint32_t pe = bf ? pack_bf16x2(e.x * inv, e.y * inv) : pack_f16x2(e.x * inv, e.y * inv);
-			if (2 * m + 1 < L) {
-				const float2 v = D[m];
-				const float orr = v.x * coef, oi = -v.y * coef;
-				const uint32_t po = bf ? pack_bf16x2(orr * inv, oi * inv) : pack_f16x2(orr * inv, oi * inv);
-				if (pairs8) {
-					reinterpret_cast<uint2*>(row)[m] = make_uint2(pe, po);
-				} else {
-					row[2 * m] = pe;
-					row[2 * m + 1] = po;
+			const float2 v = D[m];
+			my = fmaxf(my, fabsf(v.x * coef));
+			if (2 * m + 1 < L) my = fmaxf(my, fabsf(-v.y * coef));
+		}
+		for (int o = 32; o > 0; o >>= 1) {
+			my = fmaxf(my, __shfl_xor(my, o, 64));
+			xmax = fmaxf(xmax, __shfl_xor(xmax, o, 64));
+			nxmin = fmaxf(nxmin, __shfl_xor(nxmin, o, 64));
+		}
+		if ((tid & 63) == 0) {
+			redm[tid >> 6] = my;
+			redx[tid >> 6] = xmax;
+			redn[tid >> 6] = nxmin;
+		}
+		__syncthreads();
+		float gy = redm[0], gx = redx[0], gn = redn[0];
+		for (int w = 1; w < kFftThreads / 64; ++w) {
+			gy = fmaxf(gy, redm[w]);
+			gx = fmaxf(gx, redx[w]);
+			gn = fmaxf(gn, redn[w]);
+		}
+		const float gm = fmaxf(gy, fmaxf(fabsf(gx - mean), fabsf(-gn - mean)));
+		const float sc = (gm > 0.0f) ? gm : 1.0f;
+		inv = 1.0f / sc;
+		if (tid == 0) A.scale[ch] = sc;
+	}
+	const bool bf = (A.flags & kOutBf16) != 0;
+	float2* dst = A.out + static_cast<uint64_t>(ch) * L;
+	float* dmag = A.out_mag + static_cast<uint64_t>(ch) * L;
+	uint32_t* row16 = A.out16 + static_cast<uint64_t>(ch) * L;
+	// one output sample (re = x_i - mean, im = y_i) in the requested form
+	auto put1 = [&](uint32_t i, float re, float im) {
+		if (A.out16) row16[i] = bf ? pack_bf16x2(re * inv, im * inv) : pack_f16x2(re * inv, im * inv);
+		else if (mag) dmag[i] = sqrtf(re * re + im * im);
+		else dst[i] = make_float2(re, im);
+	};
+	if (up > 1 && J == kFirJ && (up == 8 || up == 4) && A.off % up == 0 && !mag) {
+		// the FIR again, lane per input position: the lane's U outputs are
+		// contiguous and aligned (off is a multiple of U), written as 16-byte
+		// stores of (x, y) pairs
+		const float* __restrict__ hg = A.hpoly;
+		const uint32_t q_lo = A.off / up, q_hi = (L - 1 + A.off) / up;
+		for (uint32_t q = q_lo + tid; q <= q_hi; q += kFftThreads) {
+			float xw[kFirJ];
+			const float* xrow = xs + J + q;
+#pragma unroll
+			for (int j = 0; j < kFirJ; ++j) xw[j] = xrow[-j];
+			const uint32_t i0 = q * up - A.off;
+			if (i0 + up <= L) {
+#pragma unroll
+				for (uint32_t h = 0; h < 8; h += 4) {
+					if (h >= up) break;
+					float re[4], im[4];
+#pragma unroll
+					for (int k = 0; k < 4; ++k) {
+						const float* hrow = hg + (h + k) * kFirJ;
+						float acc = 0.0f;
+#pragma unroll
+						for (int j = 0; j < kFirJ; ++j) acc = fmaf(hrow[j], xw[j], acc);
+						re[k] = acc - mean;
+					}
+					const float4 yv = *reinterpret_cast<const float4*>(D + ((i0 + h) >> 1));   // y pairs of 4 outputs
+					im[0] = yv.x * coef;
+					im[1] = -yv.y * coef;
+					im[2] = yv.z * coef;
+					im[3] = -yv.w * coef;
+					if (A.out16) {
+						uint4 pk;
+						if (bf) {
+							pk = make_uint4(pack_bf16x2(re[0] * inv, im[0] * inv), pack_bf16x2(re[1] * inv, im[1] * inv),
+									pack_bf16x2(re[2] * inv, im[2] * inv), pack_bf16x2(re[3] * inv, im[3] * inv));
+						} else {
+							pk = make_uint4(pack_f16x2(re[0] * inv, im[0] * inv), pack_f16x2(re[1] * inv, im[1] * inv),
+									pack_f16x2(re[2] * inv, im[2] * inv), pack_f16x2(re[3] * inv, im[3] * inv));
+						}
+						*reinterpret_cast<uint4*>(row16 + i0 + h) = pk;
+					} else {
+						*reinterpret_cast<float4*>(dst + i0 + h) = make_float4(re[0], im[0], re[1], im[1]);
+						*reinterpret_cast<float4*>(dst + i0 + h + 2) = make_float4(re[2], im[2], re[3], im[3]);
+					}
 				}
 			} else {
-				row[2 * m] = pe;
+				for (uint32_t ph = 0; ph < up; ++ph) {
+					const uint32_t i = i0 + ph;
+					if (i >= L) break;
+					const float* hrow = hg + ph * kFirJ;
+					float acc = 0.0f;
+#pragma unroll
+					for (int j = 0; j < kFirJ; ++j) acc = fmaf(hrow[j], xw[j], acc);
+					put1(i, acc - mean, hil(i));
+				}
 			}
 		}
 		return;
 	}
-	if (mag) {
-		for (uint32_t m = tid; 2 * m < L; m += kFftThreads) put(2 * m, D[m]);
-		__syncthreads();
-		for (uint32_t m = tid; m < M; m += kFftThreads) D[m] = bo[m];
-		__syncthreads();
-		fft_forward(D, A.plan, A.tw);
-		for (uint32_t m = tid; 2 * m + 1 < L; m += kFftThreads) put(2 * m + 1, D[m]);
-		return;
+	// any other filter or rate: output by output
+	for (uint32_t i = tid; i < L; i += kFftThreads) put1(i, fir_at(A, xs, hs, src, i) - mean, hil(i));
+}
+
+// ---------------------------------------------------------------- long records
+
+// Z = DFT_M(x[2m] + i x[2m+1]) -> conj(C) in place, the Hilbert spectrum of
+// the fused kernel (C[0] = 0, C[k] = i sin(2 pi k/N) Z[k] + cos(2 pi k/N)
+// conj(Z[M-k]), C[M/2] = i Z[M/2]) over rows of M complex values; lane k
+// owns the pair (k, M-k), k <= M/2.  wpost[k] = exp(-2 pi i k / N).
+__global__ void __launch_bounds__(256) hilbert_pairs_kernel(float2* __restrict__ z, uint32_t M,
+		const float2* __restrict__ wpost)
+{
+	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k > M / 2) return;
+	float2* D = z + static_cast<uint64_t>(blockIdx.y) * M;
+	if (k == 0) {
+		D[0] = make_float2(0.0f, 0.0f);
+	} else if (2 * k == M) {
+		const float2 a = D[k];
+		D[k] = make_float2(-a.y, -a.x);
+	} else {
+		const float2 a = D[k], b = D[M - k];
+		const float2 w = wpost[k];
+		const float c = w.x, sn = -w.y;
+		const float2 ck = make_float2(fmaf(c, b.x, -sn * a.y), fmaf(sn, a.x, -c * b.y));
+		const float2 cm = make_float2(fmaf(-c, a.x, -sn * b.y), fmaf(sn, b.x, c * a.y));
+		D[k] = make_float2(ck.x, -ck.y);
+		D[M - k] = make_float2(cm.x, -cm.y);
 	}
-	// F32 complex: the even samples wait in the scratch row while the odd
-	// transform runs, swapped in per index with the odd spectrum (index m is
-	// read and written by one thread only), so that every output pair
-	// (z[2m], z[2m+1]) is then written at once, 16 contiguous bytes per lane,
-	// instead of two half-filled passes over the row.
-	for (uint32_t m = tid; m < M; m += kFftThreads) {
-		const float2 od = bo[m];
-		if (2 * m < L) bo[m] = D[m];
-		D[m] = od;
+}
+
+// Analytic tensor rows from x (FIR output after DC removal, [ch][N] floats)
+// and F = DFT_M(conj C) ([ch][M] complex): z_i = (x_i, y_i) with
+// y[2m] = Re F[m] / M, y[2m+1] = -Im F[m] / M, the first L samples, in the
+// storage type (FP16/BF16 over the per-channel scale s = max(|re|, |im|), 1
+// if zero).  One workgroup per channel; lane per sample pair.
+template<int STORAGE>
+__global__ void __launch_bounds__(256) store_xy_kernel(const float* __restrict__ x, const float2* __restrict__ F,
+		uint32_t N, uint32_t L, void* __restrict__ tensor, float* __restrict__ scale, uint64_t ch0)
+{
+	__shared__ float red[4];
+	const uint32_t ch = blockIdx.x, tid = threadIdx.x, M = N / 2;
+	const float2* xr = reinterpret_cast<const float2*>(x + static_cast<uint64_t>(ch) * N);
+	const float2* fr = F + static_cast<uint64_t>(ch) * M;
+	const uint64_t gch = ch0 + ch;
+	const float coef = 2.0f / static_cast<float>(N);
+	const uint32_t npair = L / 2;
+	const bool odd = (L & 1u) != 0;   // the last sample has no partner
+	float inv = 1.0f;
+	if constexpr (STORAGE != MISTA_STORAGE_F32) {
+		float mx = 0.0f;
+		for (uint32_t m = tid; m < npair; m += 256) {
+			const float2 xv = xr[m], f = fr[m];
+			mx = fmaxf(fmaxf(mx, fmaxf(fabsf(xv.x), fabsf(xv.y))), fmaxf(fabsf(f.x * coef), fabsf(-f.y * coef)));
+		}
+		if (odd && tid == 0) mx = fmaxf(mx, fmaxf(fabsf(xr[npair].x), fabsf(fr[npair].x * coef)));
+		for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+		if ((tid & 63) == 0) red[tid >> 6] = mx;
+		__syncthreads();
+		const float gm = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+		const float sc = (gm > 0.0f) ? gm : 1.0f;
+		inv = 1.0f / sc;
+		if (tid == 0) scale[gch] = sc;
 	}
-	__syncthreads();
-	fft_forward(D, A.plan, A.tw);
-	const bool pairs16 = (L & 1u) == 0;   // rows start 16-byte aligned
-	for (uint32_t m = tid; 2 * m < L; m += kFftThreads) {
-		const float2 e = bo[m];
-		const float2 ev = make_float2(e.x * coef, -e.y * coef);
-		if (2 * m + 1 < L) {
-			const float2 o = D[m];
-			const float2 ov = make_float2(o.x * coef, -o.y * coef);
-			if (pairs16) {
-				reinterpret_cast<float4*>(dst)[m] = make_float4(ev.x, ev.y, ov.x, ov.y);
+	// rows start 16-byte (F32) / 8-byte aligned when L is even; an odd L
+	// writes sample by sample
+	for (uint32_t m = tid; m < npair; m += 256) {
+		const float2 xv = xr[m], f = fr[m];
+		const float y0 = f.x * coef, y1 = -f.y * coef;
+		if constexpr (STORAGE == MISTA_STORAGE_F32) {
+			float2* row = static_cast<float2*>(tensor) + gch * L;
+			if (!odd) {
+				reinterpret_cast<float4*>(row)[m] = make_float4(xv.x, y0, xv.y, y1);
 			} else {
-				dst[2 * m] = ev;
-				dst[2 * m + 1] = ov;
+				row[2 * m] = make_float2(xv.x, y0);
+				row[2 * m + 1] = make_float2(xv.y, y1);
 			}
 		} else {
-			dst[2 * m] = ev;
+			uint32_t* row = static_cast<uint32_t*>(tensor) + gch * L;
+			const uint32_t p0 = (STORAGE == MISTA_STORAGE_F16) ? pack_f16x2(xv.x * inv, y0 * inv) : pack_bf16x2(xv.x * inv, y0 * inv);
+			const uint32_t p1 = (STORAGE == MISTA_STORAGE_F16) ? pack_f16x2(xv.y * inv, y1 * inv) : pack_bf16x2(xv.y * inv, y1 * inv);
+			if (!odd) {
+				reinterpret_cast<uint2*>(row)[m] = make_uint2(p0, p1);
+			} else {
+				row[2 * m] = p0;
+				row[2 * m + 1] = p1;
+			}
+		}
+	}
+	if (odd && tid == 0) {
+		const float re = xr[npair].x, im = fr[npair].x * coef;
+		if constexpr (STORAGE == MISTA_STORAGE_F32) {
+			(static_cast<float2*>(tensor) + gch * L)[L - 1] = make_float2(re, im);
+		} else if constexpr (STORAGE == MISTA_STORAGE_F16) {
+			(static_cast<uint32_t*>(tensor) + gch * L)[L - 1] = pack_f16x2(re * inv, im * inv);
+		} else {
+			(static_cast<uint32_t*>(tensor) + gch * L)[L - 1] = pack_bf16x2(re * inv, im * inv);
 		}
 	}
 }
@@ -814,8 +923,13 @@ struct PrepPlan {
 	uint32_t fused_nf = 0;
 	FftPlanArgs fplan{};
 	DeviceBuffer tw, wpost;
+	// four-step path (records too long for the fused kernel)
+	Fft4Plan* f4 = nullptr;     // size Nf / 2
+	DeviceBuffer t4, wpost4;    // transpose buffer; exp(-2 pi i k / Nf), k <= Nf / 4
+	uint32_t wpost4_nf = 0;
 	~PrepPlan()
 	{
+		fft4_destroy(f4);
 		if (r2c) hipfftDestroy(r2c);
 		if (c2c) hipfftDestroy(c2c);
 		if (r2c_tail) hipfftDestroy(r2c_tail);
@@ -830,7 +944,8 @@ size_t prep_device_bytes(const PrepState& st)
 	if (!st.plan) return 0;
 	const PrepPlan& P = *st.plan;
 	// the C2C plan's work area is taken as one more complex chunk
-	return P.real.capacity() + P.spec.capacity() + 2 * P.z.capacity() + P.tw.capacity() + P.wpost.capacity();
+	return P.real.capacity() + P.spec.capacity() + 2 * P.z.capacity() + P.tw.capacity() + P.wpost.capacity() +
+			P.t4.capacity() + P.wpost4.capacity() + fft4_device_bytes(P.f4);
 }
 
 static void make_plans(hipfftHandle* r2c, hipfftHandle* c2c, uint32_t nf, uint32_t batch, hipStream_t s)
@@ -931,8 +1046,6 @@ static void prepare_fused(PrepState& st, const FrameGeom& g, const float* d_raw,
 	if (!st.plan) st.plan = new PrepPlan();
 	PrepPlan& P = *st.plan;
 	if (P.fused_nf != g.nf) build_fused_plan(P, g.nf, s);
-	const uint32_t M = g.nf / 2;
-	float2* scratch = static_cast<float2*>(P.spec.reserve(static_cast<size_t>(chunk) * M * sizeof(float2)));
 	const bool scaled = g.storage != MISTA_STORAGE_F32;
 	HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(analytic_lds_kernel),
 			hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
@@ -940,7 +1053,6 @@ static void prepare_fused(PrepState& st, const FrameGeom& g, const float* d_raw,
 	A.hpoly = d_filter_poly;
 	A.tw = P.tw.as<const float2>();
 	A.wpost = P.wpost.as<const float2>();
-	A.scratch = scratch;
 	A.ns = g.ns;
 	A.up = g.up;
 	A.J = J;
@@ -969,10 +1081,118 @@ static void prepare_fused(PrepState& st, const FrameGeom& g, const float* d_raw,
 	}
 }
 
+// FIR x U + DC removal of n channels into rows of Nf floats (zero padded).
+static void launch_fir_any(uint32_t n, const FrameGeom& g, const float* raw, const float* hpoly, uint32_t J,
+		uint32_t off, float* real, hipStream_t s)
+{
+	if (g.up > 1) {
+		const size_t lds = (static_cast<size_t>(J) + g.ns + off / g.up + 2 + static_cast<size_t>(g.up) * J) * sizeof(float);
+		if (lds > 150 * 1024) MISTA_THROW(MISTA_E_INVALID_PARAM, "channel too long for the LDS FIR: " << g.ns << " samples");
+	}
+	const uint32_t vpt = (g.L + kFirThreads - 1) / kFirThreads;
+	if (g.up == 1) {
+		hipLaunchKernelGGL(copy_dc_kernel, dim3(n), dim3(kFirThreads), 0, s, raw, g.L, g.nf, g.dz_up, real);
+	} else if (vpt <= 4) {
+		launch_fir<4>(n, g, raw, hpoly, J, off, real, s);
+	} else if (vpt <= 8) {
+		launch_fir<8>(n, g, raw, hpoly, J, off, real, s);
+	} else if (vpt <= 16) {
+		launch_fir<16>(n, g, raw, hpoly, J, off, real, s);
+	} else if (vpt <= 32) {
+		launch_fir<32>(n, g, raw, hpoly, J, off, real, s);
+	} else if (vpt <= 64) {
+		launch_fir<64>(n, g, raw, hpoly, J, off, real, s);
+	} else {
+		MISTA_THROW(MISTA_E_INVALID_PARAM, "signal length " << g.L << " exceeds 65536");
+	}
+	HIP_CHECK(hipGetLastError());
+}
+
+static void launch_store(const FrameGeom& g, uint32_t n, const float2* z, void* d_tensor, float* d_scale, uint64_t c0,
+		hipStream_t s)
+{
+	switch (g.storage) {
+	case MISTA_STORAGE_F32:
+		hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_F32>, dim3(n), dim3(256), 0, s, z, g.nf, g.L, d_tensor, d_scale, c0);
+		break;
+	case MISTA_STORAGE_F16:
+		hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_F16>, dim3(n), dim3(256), 0, s, z, g.nf, g.L, d_tensor, d_scale, c0);
+		break;
+	default:
+		hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_BF16>, dim3(n), dim3(256), 0, s, z, g.nf, g.L, d_tensor, d_scale, c0);
+		break;
+	}
+	HIP_CHECK(hipGetLastError());
+}
+
+// Records too long for the fused kernel: the fused kernel's algorithm with
+// its transforms of size M = Nf/2 as four-step FFTs through HBM (fft4.hip):
+// FIR x U + DC removal into x, Z = DFT_M(x as M complex pairs), the Hilbert
+// spectrum conj(C) in place, F = DFT_M(conj C), then z = (x, y) in the
+// storage type.  256 channels per chunk.
+static void prepare_four_step(PrepState& st, const FrameGeom& g, const float* d_raw, const float* d_filter_poly,
+		uint32_t J, uint32_t off, void* d_tensor, float* d_scale, hipStream_t s)
+{
+	const uint32_t nch = g.ntx * g.nrx;
+	const uint32_t chunk = std::min<uint32_t>(nch, 256u);
+	const uint32_t M = g.nf / 2;
+	if (!st.plan) st.plan = new PrepPlan();
+	PrepPlan& P = *st.plan;
+	if (!P.f4 || fft4_size(P.f4) != M) {
+		fft4_destroy(P.f4);
+		P.f4 = nullptr;
+		P.f4 = fft4_plan(M, s);
+	}
+	if (P.wpost4_nf != g.nf) {
+		std::vector<float2> wp(M / 2 + 1);
+		for (uint32_t k = 0; k <= M / 2; ++k) {
+			const double a = -2.0 * M_PI * static_cast<double>(k) / static_cast<double>(g.nf);
+			wp[k] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+		}
+		void* d = P.wpost4.reserve(wp.size() * sizeof(float2));
+		HIP_CHECK(hipMemcpyAsync(d, wp.data(), wp.size() * sizeof(float2), hipMemcpyHostToDevice, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		P.wpost4_nf = g.nf;
+	}
+	float* real = static_cast<float*>(P.real.reserve(static_cast<size_t>(chunk) * g.nf * sizeof(float)));
+	float2* z = static_cast<float2*>(P.z.reserve(static_cast<size_t>(chunk) * M * sizeof(float2)));
+	float2* t = static_cast<float2*>(P.t4.reserve(static_cast<size_t>(chunk) * M * sizeof(float2)));
+	for (uint32_t c0 = 0; c0 < nch; c0 += chunk) {
+		const uint32_t n = std::min(chunk, nch - c0);
+		if (st.before_chunk) st.before_chunk(c0, n);
+		launch_fir_any(n, g, d_raw + static_cast<uint64_t>(c0) * g.ns, d_filter_poly, J, off, real, s);
+		fft4_exec(*P.f4, reinterpret_cast<const float2*>(real), t, z, n, s);
+		hipLaunchKernelGGL(hilbert_pairs_kernel, dim3((M / 2 + 1 + 255) / 256, n), dim3(256), 0, s, z, M,
+				P.wpost4.as<const float2>());
+		HIP_CHECK(hipGetLastError());
+		fft4_exec(*P.f4, z, t, z, n, s);
+		switch (g.storage) {
+		case MISTA_STORAGE_F32:
+			hipLaunchKernelGGL(store_xy_kernel<MISTA_STORAGE_F32>, dim3(n), dim3(256), 0, s, real, z, g.nf, g.L, d_tensor,
+					d_scale, static_cast<uint64_t>(c0));
+			break;
+		case MISTA_STORAGE_F16:
+			hipLaunchKernelGGL(store_xy_kernel<MISTA_STORAGE_F16>, dim3(n), dim3(256), 0, s, real, z, g.nf, g.L, d_tensor,
+					d_scale, static_cast<uint64_t>(c0));
+			break;
+		default:
+			hipLaunchKernelGGL(store_xy_kernel<MISTA_STORAGE_BF16>, dim3(n), dim3(256), 0, s, real, z, g.nf, g.L, d_tensor,
+					d_scale, static_cast<uint64_t>(c0));
+			break;
+		}
+		HIP_CHECK(hipGetLastError());
+	}
+}
+
 void prepare_signals(PrepState& st, const FrameGeom& g, const float* d_raw, const float* d_filter_poly,
 		uint32_t J, uint32_t off, void* d_tensor, float* d_scale, hipStream_t s)
 {
 	const uint32_t nch = g.ntx * g.nrx;
+	if (st.method == 3) {
+		st.last_fused = false;
+		prepare_four_step(st, g, d_raw, d_filter_poly, J, off, d_tensor, d_scale, s);
+		return;
+	}
 	if (st.method != 1) {
 		const size_t lds = fused_lds_bytes(g, J, off);
 		if (lds) {
@@ -981,6 +1201,9 @@ void prepare_signals(PrepState& st, const FrameGeom& g, const float* d_raw, cons
 			return;
 		}
 		if (st.method == 2) MISTA_THROW(MISTA_E_INVALID_PARAM, "the fused preparation does not fit this frame (FFT size " << g.nf << ")");
+		st.last_fused = false;
+		prepare_four_step(st, g, d_raw, d_filter_poly, J, off, d_tensor, d_scale, s);
+		return;
 	}
 	st.last_fused = false;
 	const uint32_t chunk = std::min<uint32_t>(nch, 1024u);
@@ -1009,31 +1232,10 @@ void prepare_signals(PrepState& st, const FrameGeom& g, const float* d_raw, cons
 	float2* spec = static_cast<float2*>(P.spec.reserve(static_cast<size_t>(chunk) * (g.nf / 2 + 1) * sizeof(float2)));
 	float2* z = static_cast<float2*>(P.z.reserve(static_cast<size_t>(chunk) * g.nf * sizeof(float2)));
 
-	if (g.up > 1) {
-		const size_t lds = (static_cast<size_t>(J) + g.ns + off / g.up + 2 + static_cast<size_t>(g.up) * J) * sizeof(float);
-		if (lds > 150 * 1024) MISTA_THROW(MISTA_E_INVALID_PARAM, "channel too long for the LDS FIR: " << g.ns << " samples");
-	}
-	const uint32_t vpt = (g.L + kFirThreads - 1) / kFirThreads;
-	if (vpt > 64) MISTA_THROW(MISTA_E_INVALID_PARAM, "signal length " << g.L << " exceeds 65536");
-
 	for (uint32_t c0 = 0; c0 < nch; c0 += chunk) {
 		const uint32_t n = std::min(chunk, nch - c0);
 		if (st.before_chunk) st.before_chunk(c0, n);
-		const float* raw = d_raw + static_cast<uint64_t>(c0) * g.ns;
-		if (g.up == 1) {
-			hipLaunchKernelGGL(copy_dc_kernel, dim3(n), dim3(kFirThreads), 0, s, raw, g.L, g.nf, g.dz_up, real);
-		} else if (vpt <= 4) {
-			launch_fir<4>(n, g, raw, d_filter_poly, J, off, real, s);
-		} else if (vpt <= 8) {
-			launch_fir<8>(n, g, raw, d_filter_poly, J, off, real, s);
-		} else if (vpt <= 16) {
-			launch_fir<16>(n, g, raw, d_filter_poly, J, off, real, s);
-		} else if (vpt <= 32) {
-			launch_fir<32>(n, g, raw, d_filter_poly, J, off, real, s);
-		} else {
-			launch_fir<64>(n, g, raw, d_filter_poly, J, off, real, s);
-		}
-		HIP_CHECK(hipGetLastError());
+		launch_fir_any(n, g, d_raw + static_cast<uint64_t>(c0) * g.ns, d_filter_poly, J, off, real, s);
 		hipfftHandle r2c = (n == chunk) ? P.r2c : P.r2c_tail;
 		hipfftHandle c2c = (n == chunk) ? P.c2c : P.c2c_tail;
 		FFT_CHECK(hipfftExecR2C(r2c, real, reinterpret_cast<hipfftComplex*>(spec)));
@@ -1041,21 +1243,7 @@ void prepare_signals(PrepState& st, const FrameGeom& g, const float* d_raw, cons
 		HIP_CHECK(hipGetLastError());
 		FFT_CHECK(hipfftExecC2C(c2c, reinterpret_cast<hipfftComplex*>(z), reinterpret_cast<hipfftComplex*>(z),
 				HIPFFT_BACKWARD));
-		switch (g.storage) {
-		case MISTA_STORAGE_F32:
-			hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_F32>, dim3(n), dim3(256), 0, s, z, g.nf, g.L, d_tensor, d_scale,
-					static_cast<uint64_t>(c0));
-			break;
-		case MISTA_STORAGE_F16:
-			hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_F16>, dim3(n), dim3(256), 0, s, z, g.nf, g.L, d_tensor, d_scale,
-					static_cast<uint64_t>(c0));
-			break;
-		default:
-			hipLaunchKernelGGL(store_kernel<MISTA_STORAGE_BF16>, dim3(n), dim3(256), 0, s, z, g.nf, g.L, d_tensor, d_scale,
-					static_cast<uint64_t>(c0));
-			break;
-		}
-		HIP_CHECK(hipGetLastError());
+		launch_store(g, n, z, d_tensor, d_scale, c0, s);
 	}
 }
 
@@ -1117,15 +1305,12 @@ void envelope_rows(PrepState& st, float* d_data, uint32_t n1, uint32_t n2, hipSt
 	if (!st.plan) st.plan = new PrepPlan();
 	PrepPlan& P = *st.plan;
 	if (P.fused_nf != g.nf) build_fused_plan(P, g.nf, s);
-	const uint32_t M = g.nf / 2;
 	const uint32_t chunk = std::min<uint32_t>(n1, 1024u);
-	float2* scratch = static_cast<float2*>(P.spec.reserve(static_cast<size_t>(chunk) * M * sizeof(float2)));
 	HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(analytic_lds_kernel),
 			hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
 	AnalyticArgs A{};
 	A.tw = P.tw.as<const float2>();
 	A.wpost = P.wpost.as<const float2>();
-	A.scratch = scratch;
 	A.ns = n2;
 	A.up = 1;
 	A.L = n2;
@@ -1136,7 +1321,9 @@ void envelope_rows(PrepState& st, float* d_data, uint32_t n1, uint32_t n2, hipSt
 		const uint32_t n = std::min(chunk, n1 - r0);
 		A.raw = d_data + static_cast<uint64_t>(r0) * n2;
 		A.out = nullptr;
-		A.out_mag = d_data + static_cast<uint64_t>(r0) * n2;   // in place: a row is in LDS before it is written
+		// in place: each sample is read (for the transform and again for the
+		// output) by the thread that later writes its magnitude
+		A.out_mag = d_data + static_cast<uint64_t>(r0) * n2;
 		hipLaunchKernelGGL(analytic_lds_kernel, dim3(n), dim3(kFftThreads), lds, s, A);
 		HIP_CHECK(hipGetLastError());
 	}

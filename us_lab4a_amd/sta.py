@@ -152,7 +152,8 @@ class HipSTAProcessor:
         self._grid_xz = xz
 
     def set_prep(self, method):
-        """0 automatic, 1 library FFT (hipFFT), 2 fused LDS kernel."""
+        """0 automatic, 1 library FFT (hipFFT), 2 fused LDS kernel,
+        3 four-step FFTs (long records)."""
         check(lib().mista_set_prep(self._h, method), self._h)
 
     def set_kernel(self, variant, p0=0, p1=0):
