@@ -49,8 +49,8 @@ def parse():
                     help="frame: the headline line; batch: FP16 frames through the serving engine vs plain "
                          "FP32 frames; serve: Poisson arrivals of a mixed-shape job stream")
     ap.add_argument("--inflight", type=int, default=4)
-    ap.add_argument("--rates", default="50,200,1000", help="serve mode: arrival rates (frames/s)")
-    ap.add_argument("--jobs", type=int, default=48, help="serve mode: jobs per rate")
+    ap.add_argument("--rates", default="5,20,50,200,1000,2000", help="serve mode: arrival rates (frames/s)")
+    ap.add_argument("--jobs", type=int, default=160, help="serve mode: most jobs per rate (at most 8 s of arrivals)")
     return ap.parse_args()
 
 
@@ -252,8 +252,13 @@ def batch_mode(args):
 def serve_mode(args):
     """Poisson arrivals of 2-D jobs with seeded uneven shapes (elements in
     {32,64,128,192,256}, samples in {2048,4096,8192}, grids 128x128 ..
-    512x1024, coherence none/PCF/SCF), FP16 storage, U = 8: p50/p99 latency
-    and aggregate Gpixel*pair/s per arrival rate."""
+    512x1024, coherence none/PCF/SCF), FP16 storage, U = 8, through the
+    serving engine.  Every shape is built, tuned and warmed before the
+    measured windows; each window reports how many handles were built and
+    tuners run inside it (both should be 0).  Per arrival rate: jobs served
+    and refused, frames/s, Gpixel*pair/s, p50/p99 latency, and latency over
+    the job's own device time alone; the capacity is the back-to-back rate
+    of the same jobs (their summed device time alone)."""
     import torch
     from us_lab4a_amd import workloads, _lib
     from us_lab4a_amd.serve import Engine, Job2D
@@ -278,46 +283,87 @@ def serve_mode(args):
         catalog.append((wl, d))
         print(f"[bench] shape {i}: {N} el, {ns} samples, grid {gx}x{gz}, {wl.cf}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
+    outs = {}
+
+    def job(k, slot):
+        wl, d = catalog[k]
+        npix = wl.n_x * wl.n_z
+        key = (k, slot % 256)   # distinct outputs for every job that can be queued or in flight
+        if key not in outs:
+            outs[key] = (torch.empty(npix, device="cuda"), torch.empty(npix, device="cuda"))
+        v, f = outs[key]
+        return wl, Job2D(config=wl.sta_config(), upsampling=8, coherence=CoherenceFactor(wl.cf, wl.cf_param),
+                         grid=wl.grid(), rx_apod=wl.rx_apodization(), echoes=d, value=v, factor=f, storage="f16")
+
     results = []
-    with Engine(max_inflight=args.inflight, max_queued=1024) as eng:
-        def job(k):
-            wl, d = catalog[k]
-            npix = wl.n_x * wl.n_z
-            return wl, Job2D(config=wl.sta_config(), upsampling=8, coherence=CoherenceFactor(wl.cf, wl.cf_param),
-                             grid=wl.grid(), rx_apod=wl.rx_apodization(), echoes=d,
-                             value=torch.empty(npix, device="cuda"), factor=torch.empty(npix, device="cuda"),
-                             storage="f16")
-        for k in range(len(catalog)):   # warm the resident handles
-            assert eng.wait(eng.submit(job(k)[1]))["ok"]
+    with Engine(max_inflight=args.inflight, max_queued=1024, handles_per_shape=2) as eng:
+        # build, tune and warm every shape's handles, then time each job alone
+        for k in range(len(catalog)):
+            assert eng.wait(eng.submit(job(k, 0)[1]))["ok"]
+        assert eng.idle()
+        alone_ms, tiles = [], []
+        for k in range(len(catalog)):
+            ms = []
+            for r in range(3):
+                s = eng.wait(eng.submit(job(k, r)[1]))
+                assert s["ok"]
+                ms.append(s["device_ms"])
+            alone_ms.append(float(np.median(ms)))
+            tiles.append(s["das_tile"])
+            print(f"[bench] shape {k} alone: {alone_ms[-1]:.2f} ms, DAS tile {s['das_tile']}", file=sys.stderr,
+                  flush=True)
+        warm = eng.counters()
+        print(f"[bench] warm engine: {warm}", file=sys.stderr, flush=True)
         for rate in [float(r) for r in args.rates.split(",")]:
-            picks = rng.integers(len(catalog), size=args.jobs)
-            gaps = rng.exponential(1.0 / rate, size=args.jobs)
-            pending, pairs, lat, busy = [], 0, [], 0
+            n_jobs = max(16, min(args.jobs, int(rate * 8)))
+            picks = rng.integers(len(catalog), size=n_jobs)
+            gaps = rng.exponential(1.0 / rate, size=n_jobs)
+            b2b_ms = float(sum(alone_ms[int(k)] for k in picks))
+            capacity = n_jobs / (b2b_ms / 1e3)
+            c0 = eng.counters()
+            pending, pairs, busy = [], 0, 0
             t0 = time.perf_counter()
             t_next = t0
-            for k, g in zip(picks, gaps):
+            for n, (k, g) in enumerate(zip(picks, gaps)):
                 t_next += g
                 while time.perf_counter() < t_next:
                     time.sleep(min(1e-3, max(0.0, t_next - time.perf_counter())))
-                wl, j = job(int(k))
+                wl, j = job(int(k), n)
                 try:
-                    pending.append((wl, j, eng.submit(j)))
+                    pending.append((int(k), wl, j, eng.submit(j)))
                 except _lib.BusyError:
                     busy += 1
-            for wl, j, t in pending:
+            lat, ratio = [], []
+            for k, wl, j, t in pending:
                 s = eng.wait(t)
+                assert s["ok"], s
                 lat.append(s["latency_ms"])
+                ratio.append(s["latency_ms"] / alone_ms[k])
                 pairs += wl.pixel_pairs
             wall = time.perf_counter() - t0
-            results.append({"rate_fps": rate, "jobs": len(pending), "refused": busy,
-                            "gpixel_pair_per_s": round(pairs / wall / 1e9, 2),
-                            "frames_per_s": round(len(pending) / wall, 2),
-                            "p50_ms": round(float(np.percentile(lat, 50)), 2),
-                            "p99_ms": round(float(np.percentile(lat, 99)), 2)})
-            print(f"[bench] rate {rate}: {results[-1]}", file=sys.stderr, flush=True)
+            c1 = eng.counters()
+            fps = len(pending) / wall
+            below = rate < 0.8 * capacity
+            r = {"rate_fps": rate, "jobs": len(pending), "refused": busy,
+                 "frames_per_s": round(fps, 2), "gpixel_pair_per_s": round(pairs / wall / 1e9, 2),
+                 "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                 "p50_latency_over_device_alone": round(float(np.percentile(ratio, 50)), 2),
+                 "back_to_back_frames_per_s": round(capacity, 2),
+                 "frac_of_back_to_back": round(fps / capacity, 3),
+                 "regime": "below capacity" if below else "saturated",
+                 "handles_built_in_window": c1["handles_created"] - c0["handles_created"],
+                 "tunes_in_window": c1["tunes"] - c0["tunes"]}
+            results.append(r)
+            print(f"[bench] rate {rate}: {r}", file=sys.stderr, flush=True)
+        final = eng.counters()
     print(json.dumps({"metric": "serving mix: Gpixel*pair/s and p50/p99 latency per arrival rate",
-                      "unit": "Gpixel*pair/s", "n_gpus": 1, "inflight": args.inflight, "storage": "f16",
-                      "data": "synthetic", "rates": results}), flush=True)
+                      "unit": "Gpixel*pair/s", "n_gpus": 1, "inflight": args.inflight, "handles_per_shape": 2,
+                      "storage": "f16", "data": "synthetic",
+                      "shapes": [{"elements": wl.num_elements, "samples": wl.num_samples, "grid": [wl.n_x, wl.n_z],
+                                  "coherence": wl.cf, "device_ms_alone": round(alone_ms[k], 3),
+                                  "das_kernel": "gather" if tiles[k] < 0 else f"tiled (shape {tiles[k]})"}
+                                 for k, (wl, _) in enumerate(catalog)],
+                      "engine_counters": final, "rates": results}), flush=True)
 
 
 def main():

@@ -65,7 +65,7 @@ class EngineConfig(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("max_inflight", ctypes.c_uint32),
         ("max_queued", ctypes.c_uint32),
-        ("pad_", ctypes.c_uint32),
+        ("handles_per_shape", ctypes.c_uint32),
         ("hbm_budget", ctypes.c_uint64),
     ]
 
@@ -93,7 +93,7 @@ class JobStats(ctypes.Structure):
     """mista_job_stats (include/mista.h)."""
     _fields_ = [
         ("status", ctypes.c_int32),
-        ("pad_", ctypes.c_uint32),
+        ("das_tile", ctypes.c_int32),
         ("queued_ms", ctypes.c_double),
         ("latency_ms", ctypes.c_double),
         ("device_ms", ctypes.c_double),
@@ -172,6 +172,8 @@ SIGNATURES = {
     "mista_engine_wait": (_i32, [_vp, _u64, ctypes.POINTER(JobStats)]),
     "mista_engine_query": (_i32, [_vp, _u64, ctypes.POINTER(ctypes.c_int)]),
     "mista_engine_stats": (_i32, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "mista_engine_counters": (_i32, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64), ctypes.POINTER(_u64),
+                                     ctypes.POINTER(_u32)]),
     "mista_engine_last_error": (ctypes.c_char_p, [_vp]),
     "mista_create_3d": (_i32, [ctypes.POINTER(Sta3dConfig), _vp, _vp, ctypes.POINTER(_vp)]),
     "mista_set_elements_3d": (_i32, [_vp, _vp, _vp]),

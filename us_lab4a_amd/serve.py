@@ -4,10 +4,13 @@
         t = eng.submit(job)            # Job2D: configuration, grid, apodization, echoes, outputs
         stats = eng.wait(t)            # JobStats: status, queued/latency/device ms, footprint
 
-Jobs of uneven shapes are admitted in arrival order as in-flight slots and
-device memory allow, run on resident processor handles (delay tables, window
-tables and FFT plans kept in HBM per shape) and overlap on the device.  The
-images of a job do not depend on the other jobs in flight.
+Jobs of uneven shapes are admitted as in-flight slots and device memory
+allow (arrival order, except that a job that can start may pass one that must
+wait), run on resident processor handles (delay tables, window tables, tuned
+tile shape and FFT plans kept in HBM per shape; `handles_per_shape` handles
+per shape so that one frame's preparation overlaps the previous DAS) and
+overlap on the device.  Handles are built on the engine's builder thread.
+The images of a job do not depend on the other jobs in flight.
 """
 import ctypes
 from dataclasses import dataclass
@@ -90,12 +93,13 @@ class Job2D:
 
 
 class Engine:
-    def __init__(self, max_inflight=4, max_queued=256, hbm_budget=0, device=0):
+    def __init__(self, max_inflight=4, max_queued=256, hbm_budget=0, device=0, handles_per_shape=0):
         c = _lib.EngineConfig()
         c.device = device
         c.max_inflight = max_inflight
         c.max_queued = max_queued
         c.hbm_budget = hbm_budget
+        c.handles_per_shape = handles_per_shape
         h = ctypes.c_void_p()
         rc = lib().mista_engine_create(ctypes.byref(c), ctypes.byref(h))
         if rc != _lib.MISTA_OK:
@@ -121,7 +125,7 @@ class Engine:
         st = _lib.JobStats()
         rc = lib().mista_engine_wait(self._e, ticket, ctypes.byref(st))
         self._jobs.pop(ticket, None)
-        return {"status": st.status, "queued_ms": st.queued_ms, "latency_ms": st.latency_ms,
+        return {"status": st.status, "das_tile": st.das_tile, "queued_ms": st.queued_ms, "latency_ms": st.latency_ms,
                 "device_ms": st.device_ms, "footprint": st.footprint, "ok": rc == _lib.MISTA_OK}
 
     def done(self, ticket):
@@ -135,6 +139,21 @@ class Engine:
         r, i, q = ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32()
         lib().mista_engine_stats(self._e, ctypes.byref(r), ctypes.byref(i), ctypes.byref(q))
         return {"resident_bytes": r.value, "inflight": i.value, "queued": q.value}
+
+    def counters(self):
+        """Totals since creation: handles built, autotuner runs, evictions;
+        and handles being built now."""
+        h, t, e, b = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint32()
+        lib().mista_engine_counters(self._e, ctypes.byref(h), ctypes.byref(t), ctypes.byref(e), ctypes.byref(b))
+        return {"handles_created": h.value, "tunes": t.value, "evictions": e.value, "building": b.value}
+
+    def idle(self, timeout_s=600.0):
+        """Waits until no handle is being built (after a warm-up)."""
+        import time
+        t0 = time.perf_counter()
+        while self.counters()["building"] and time.perf_counter() - t0 < timeout_s:
+            time.sleep(0.005)
+        return self.counters()["building"] == 0
 
     def close(self):
         if getattr(self, "_e", None):
