@@ -242,7 +242,10 @@ def test_autotuned_kernel_equals_gather_kernel(small):
         if variant == 0:
             tuned = p.tuned_kernel()
             say("autotuner chose", tuned)
-            assert tuned[0] in (1, 2, 3, 5, 6) and (tuned[1] >> 16) in (52, 76, 150) and (tuned[1] & 0xFF) in (4, 8)
+            # a shape with a budget of 52, 76 or 150 KiB and 4 or 8 pairs per
+            # chunk, or (0) the smallest budget in which the windows fit
+            assert tuned[0] in (1, 2, 3, 5, 6) and (tuned[1] == 0 or (
+                (tuned[1] >> 16) in (52, 76, 150) and (tuned[1] & 0xFF) in (4, 8)))
     for v, f in outs[:2]:
         assert np.array_equal(v, outs[2][0]) and np.array_equal(f, outs[2][1])
     p.close()

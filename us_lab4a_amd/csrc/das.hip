@@ -497,10 +497,13 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 					// untimed for comparison (it builds the shape's tables), but
 					// a first run this long ends the search with its own result
 					HIP_CHECK(hipEventRecord(e0, s));
-					const int tries[3] = {8 | (kbs[0] << 16), 4 | (kbs[0] << 16), kbs[1] << 16};
+					// the last try takes the smallest LDS budget the windows
+					// fit, up to a CU's whole LDS (long windows: steep angles,
+					// coarse grids)
+					const int tries[4] = {8 | (kbs[0] << 16), 4 | (kbs[0] << 16), kbs[1] << 16, 0};
 					int built = 0;
 					bool ok = false;
-					for (int k = 0; k < 3 && !ok; ++k) {
+					for (int k = 0; k < 4 && !ok; ++k) {
 						built = tries[k];
 						ok = launch_tiled(g, p0, built, scratch, a, d_delay, s);
 					}
@@ -516,22 +519,26 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 						bp1 = built;
 						break;
 					}
+					// pairs per chunk (8 first: the better one when it fits) in
+					// each budget, then the smallest budget that fits
+					std::vector<int> cands;
 					for (int kb : kbs)
-						for (int tc : {8, 4}) {   // pairs per chunk (8 first: the better one when it fits)
-							if (bp0 >= 0 && best > kTuneLimitMs) break;
-							const int p1 = tc | (kb << 16);
-							HIP_CHECK(hipEventRecord(e0, s));
-							const bool run = launch_tiled(g, p0, p1, scratch, a, d_delay, s);
-							HIP_CHECK(hipEventRecord(e1, s));
-							HIP_CHECK(hipEventSynchronize(e1));
-							float ms = 0.0f;
-							HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-							if (run && (bp0 < 0 || ms < best)) {
-								best = ms;
-								bp0 = p0;
-								bp1 = p1;
-							}
+						for (int tc : {8, 4}) cands.push_back(tc | (kb << 16));
+					cands.push_back(0);
+					for (const int p1 : cands) {
+						if (bp0 >= 0 && best > kTuneLimitMs) break;
+						HIP_CHECK(hipEventRecord(e0, s));
+						const bool run = launch_tiled(g, p0, p1, scratch, a, d_delay, s);
+						HIP_CHECK(hipEventRecord(e1, s));
+						HIP_CHECK(hipEventSynchronize(e1));
+						float ms = 0.0f;
+						HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+						if (run && (bp0 < 0 || ms < best)) {
+							best = ms;
+							bp0 = p0;
+							bp1 = p1;
 						}
+					}
 				}
 				(void)hipEventDestroy(e0);
 				(void)hipEventDestroy(e1);

@@ -376,27 +376,46 @@ __device__ __forceinline__ void wait_vmcnt_at_most(uint32_t n)
 	else MISTA_VMCNT(0);
 }
 
-// Four LDS reads of two consecutive entries each (byte addresses addr),
-// issued back to back and waited for once, in one asm block: the compiler
+// N (4 or 8) LDS reads of two consecutive entries each (byte addresses
+// addr), issued back to back and waited for once, in one asm block: the compiler
 // would otherwise wait for each read before the next.  The outputs are
 // early-clobbered so no address register is reused while a read is in flight.
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
-template<uint32_t ESZ>
-__device__ __forceinline__ void lds_read_pairs4(const uint32_t* addr, typename LdsPair<ESZ>::T* out)
+#define MISTA_RD2(op, o, a) op " %" #o ", %" #a " offset1:1\n\t"
+template<uint32_t ESZ, int N>
+__device__ __forceinline__ void lds_read_pairs(const uint32_t* addr, typename LdsPair<ESZ>::T* out)
 {
-	if constexpr (ESZ == 8) {
-		asm volatile("ds_read2_b64 %0, %4 offset1:1\n\tds_read2_b64 %1, %5 offset1:1\n\t"
-				"ds_read2_b64 %2, %6 offset1:1\n\tds_read2_b64 %3, %7 offset1:1\n\ts_waitcnt lgkmcnt(0)"
+	static_assert(N == 4 || N == 8, "four or eight reads");
+	if constexpr (N == 4 && ESZ == 8) {
+		asm volatile(MISTA_RD2("ds_read2_b64", 0, 4) MISTA_RD2("ds_read2_b64", 1, 5) MISTA_RD2("ds_read2_b64", 2, 6)
+				MISTA_RD2("ds_read2_b64", 3, 7) "s_waitcnt lgkmcnt(0)"
 				: "=&v"(out[0]), "=&v"(out[1]), "=&v"(out[2]), "=&v"(out[3])
 				: "v"(addr[0]), "v"(addr[1]), "v"(addr[2]), "v"(addr[3]) : "memory");
+	} else if constexpr (N == 4) {
+		asm volatile(MISTA_RD2("ds_read2_b32", 0, 4) MISTA_RD2("ds_read2_b32", 1, 5) MISTA_RD2("ds_read2_b32", 2, 6)
+				MISTA_RD2("ds_read2_b32", 3, 7) "s_waitcnt lgkmcnt(0)"
+				: "=&v"(out[0]), "=&v"(out[1]), "=&v"(out[2]), "=&v"(out[3])
+				: "v"(addr[0]), "v"(addr[1]), "v"(addr[2]), "v"(addr[3]) : "memory");
+	} else if constexpr (ESZ == 8) {
+		asm volatile(MISTA_RD2("ds_read2_b64", 0, 8) MISTA_RD2("ds_read2_b64", 1, 9) MISTA_RD2("ds_read2_b64", 2, 10)
+				MISTA_RD2("ds_read2_b64", 3, 11) MISTA_RD2("ds_read2_b64", 4, 12) MISTA_RD2("ds_read2_b64", 5, 13)
+				MISTA_RD2("ds_read2_b64", 6, 14) MISTA_RD2("ds_read2_b64", 7, 15) "s_waitcnt lgkmcnt(0)"
+				: "=&v"(out[0]), "=&v"(out[1]), "=&v"(out[2]), "=&v"(out[3]), "=&v"(out[4]), "=&v"(out[5]),
+				  "=&v"(out[6]), "=&v"(out[7])
+				: "v"(addr[0]), "v"(addr[1]), "v"(addr[2]), "v"(addr[3]), "v"(addr[4]), "v"(addr[5]), "v"(addr[6]),
+				  "v"(addr[7]) : "memory");
 	} else {
-		asm volatile("ds_read2_b32 %0, %4 offset1:1\n\tds_read2_b32 %1, %5 offset1:1\n\t"
-				"ds_read2_b32 %2, %6 offset1:1\n\tds_read2_b32 %3, %7 offset1:1\n\ts_waitcnt lgkmcnt(0)"
-				: "=&v"(out[0]), "=&v"(out[1]), "=&v"(out[2]), "=&v"(out[3])
-				: "v"(addr[0]), "v"(addr[1]), "v"(addr[2]), "v"(addr[3]) : "memory");
+		asm volatile(MISTA_RD2("ds_read2_b32", 0, 8) MISTA_RD2("ds_read2_b32", 1, 9) MISTA_RD2("ds_read2_b32", 2, 10)
+				MISTA_RD2("ds_read2_b32", 3, 11) MISTA_RD2("ds_read2_b32", 4, 12) MISTA_RD2("ds_read2_b32", 5, 13)
+				MISTA_RD2("ds_read2_b32", 6, 14) MISTA_RD2("ds_read2_b32", 7, 15) "s_waitcnt lgkmcnt(0)"
+				: "=&v"(out[0]), "=&v"(out[1]), "=&v"(out[2]), "=&v"(out[3]), "=&v"(out[4]), "=&v"(out[5]),
+				  "=&v"(out[6]), "=&v"(out[7])
+				: "v"(addr[0]), "v"(addr[1]), "v"(addr[2]), "v"(addr[3]), "v"(addr[4]), "v"(addr[5]), "v"(addr[6]),
+				  "v"(addr[7]) : "memory");
 	}
 }
+#undef MISTA_RD2
 #pragma clang diagnostic pop
 
 // Chunk i of a tile = pairs (rx r, tx c*TC .. c*TC+TC-1), r = i / CPR,
@@ -410,8 +429,15 @@ __device__ __forceinline__ void lds_read_pairs4(const uint32_t* addr, typename L
 // weights are scalar-loaded one chunk ahead; its TC LDS reads are issued
 // together and waited for once.  Chunks in which no pair can reach a record
 // end (the common case) take a path without the range test.
+// Occupancy: 256-lane workgroups target three waves per SIMD (up to 170
+// registers each); a 512-lane workgroup with its large LDS ring holds a CU
+// alone (two waves per SIMD), so it may use up to 256 registers, which buys
+// all of a chunk's LDS reads in flight at once.
+template<int NTHREADS> constexpr int tile_waves_per_eu() { return NTHREADS >= 512 ? 2 : 3; }
+
 template<int STORAGE, int CF, int NTXMAX, int TC, int NTHREADS, bool MULTI>
-__global__ void __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(3, 3)))
+__global__ void __launch_bounds__(NTHREADS)
+__attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<NTHREADS>(), tile_waves_per_eu<NTHREADS>())))
 das_tile_kernel(const TileArgs A)
 {
 	using E = typename Entry<STORAGE>::T;
@@ -422,9 +448,11 @@ das_tile_kernel(const TileArgs A)
 	static_assert(TC % NW == 0, "every wave copies whole windows");
 	constexpr uint32_t PPW = TC / NW;       // windows copied per wave per chunk
 	constexpr uint32_t CPB = NTXMAX / TC;   // chunks per block of NTXMAX transmits
-	// LDS reads issued together: four where registers allow, one at a time
-	// when 128 registers hold transmit delays
-	constexpr int BATCH = NTXMAX <= 64 ? 4 : 1;
+	// LDS reads issued together: a whole chunk's where the register budget
+	// allows it (two waves per SIMD), four with up to 64 transmit delays in
+	// registers, else one at a time (128 registers hold transmit delays)
+	constexpr int BATCH = NTHREADS >= 512 ? TC : (NTXMAX <= 64 ? 4 : 1);
+	static_assert(BATCH == 1 || TC % BATCH == 0, "whole read groups per chunk");
 	extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
 
 	const DasArgs& a = A.d;
@@ -611,18 +639,18 @@ das_tile_kernel(const TileArgs A)
 					ai = fmaf(v.y, wv, ai);
 				};
 				auto run = [&](auto&& pair) {
-					if constexpr (BATCH == 4) {
-						// groups of four pairs: four LDS reads in flight
+					if constexpr (BATCH > 1) {
+						// groups of BATCH pairs: BATCH LDS reads in flight
 #pragma unroll
-						for (int g4 = 0; g4 < TC; g4 += 4) {
-							uint32_t addr[4];
-							float kf[4], wv[4];
+						for (int g = 0; g < TC; g += BATCH) {
+							uint32_t addr[BATCH];
+							float kf[BATCH], wv[BATCH];
 #pragma unroll
-							for (int q = 0; q < 4; ++q) pair(g4 + q, addr[q], kf[q], wv[q]);
-							typename LdsPair<ESZ>::T e[4];
-							lds_read_pairs4<ESZ>(addr, e);
+							for (int q = 0; q < BATCH; ++q) pair(g + q, addr[q], kf[q], wv[q]);
+							typename LdsPair<ESZ>::T e[BATCH];
+							lds_read_pairs<ESZ, BATCH>(addr, e);
 #pragma unroll
-							for (int q = 0; q < 4; ++q) accumulate(Entry<STORAGE>::lerp2(e[q], kf[q]), wv[q]);
+							for (int q = 0; q < BATCH; ++q) accumulate(Entry<STORAGE>::lerp2(e[q], kf[q]), wv[q]);
 						}
 					} else {
 #pragma unroll
