@@ -454,7 +454,7 @@ def main():
     value = wl.pixel_pairs / (ms_per_step * 1e-3) / 1e9
     cols = [0, 137, 255, 256, 400, 511] if wl.n_x == 512 else list(np.linspace(0, wl.n_x - 1, 6).astype(int))
     images = {args.storage: d_value.cpu().numpy().reshape(wl.n_x, wl.n_z)}
-    tuned = proc.tuned_kernel()
+    ran = proc.last_kernel()   # the configuration the timed frames ran: shape, chunk size, ring depth
     proc.close()
     headline = None
     if not args.no_headline:
@@ -487,7 +487,7 @@ def main():
         "config": {"workload": wl.name, "elements": wl.num_elements, "tx": wl.num_tx, "samples": wl.num_samples,
                    "upsampling": wl.upsampling, "grid": [wl.n_x, wl.n_z], "coherence": wl.cf,
                    "storage": args.storage, "pixel_pairs_per_frame": wl.pixel_pairs,
-                   "das_tile": tile_name(tuple(args.tile) if args.tile else tuned),
+                   "das_tile": tile_name(ran),
                    "graph": bool(args.graph)},
         "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                              "p99": round(float(np.percentile(lat, 99)), 4)},
@@ -547,15 +547,16 @@ def instruction_bounds(wl, storage, das_ms):
             "launch_ms": round(das_ms, 3), "source": os.path.relpath(path, ROOT), "per_64_pixel_pairs": per}
 
 
-def tile_name(tuned):
-    """Human-readable tile choice of the DAS autotuner."""
-    p0, p1 = tuned
+def tile_name(ran):
+    """Human-readable DAS configuration (mista_last_kernel: tile shape; chunk
+    size, ring buffers and LDS budget)."""
+    p0, p1 = ran
     if p0 < 0:
         return "gather kernel"
     shape = {2: "8x32", 3: "32x8", 4: "8x8x4", 5: "16x32", 6: "32x16", 7: "8x8x8"}.get(p0, "16x16")
-    kb = (p1 >> 16) & 0xFF
-    tc = p1 & 0xFF
-    return f"{shape} tile, {kb or 'default'} KiB LDS ring, {tc or 'default'} pairs per chunk"
+    tc, nb, kb = p1 & 0xFF, (p1 >> 8) & 0xFF, (p1 >> 16) & 0xFF
+    budget = f"{kb} KiB budget" if kb else "smallest fitting budget"
+    return f"{shape} tile, {tc} pairs per chunk, {nb} ring buffers ({budget})"
 
 
 if __name__ == "__main__":

@@ -185,6 +185,18 @@ int mista_prepare_signals_device(mista_handle* h, const float* d_echoes, uint32_
 /* Delay-and-sum only over the handle's resident tensor. */
 int mista_das_device(mista_handle* h, float* d_value, float* d_factor, void* stream);
 
+/* No counterpart in the reference (frames of NetworkSyncSTAMethod.cpp:93-140
+ * each run their own processor): the DAS of n prepared handles, up to 16
+ * frames in one launch of the tiled kernel when their launches agree in
+ * storage, coherence factor, transmit block and workgroup size and their
+ * geometries have been tuned (a frame each handle ran through
+ * mista_das_device/mista_process_device); others run one by one.  Every
+ * frame writes the images it writes alone.  Work goes on `stream` (NULL:
+ * the first handle's), which must be ordered after each handle's
+ * preparation.  mista_last_timing reports nothing for these frames. */
+int mista_das_batch_device(mista_handle* const* handles, uint32_t n, float* const* d_values, float* const* d_factors,
+		void* stream);
+
 /* No counterpart in the reference: mista_das_device for callers that cannot
  * allocate device memory (code under stream capture, tight serving loops,
  * harnesses that forbid hipMalloc).  Same inputs and outputs and the same

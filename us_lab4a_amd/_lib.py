@@ -183,6 +183,7 @@ SIGNATURES = {
     "mista_prepare_signals_device": (_i32, [_vp, _vp, _u32, _vp]),
     "mista_das_device": (_i32, [_vp, _vp, _vp, _vp]),
     "mista_das_device_noalloc": (_i32, [_vp, _vp, _vp, _vp]),
+    "mista_das_batch_device": (_i32, [_vp, _u32, _vp, _vp, _vp]),
     "mista_upload_analytic": (_i32, [_vp, _vp, _u32]),
     "mista_download_analytic": (_i32, [_vp, _vp]),
     "mista_download_delays": (_i32, [_vp, _vp]),

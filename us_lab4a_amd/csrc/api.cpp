@@ -756,6 +756,39 @@ int mista_das_device(mista_handle* h, float* d_value, float* d_factor, void* str
 	});
 }
 
+int mista_das_batch_device(mista_handle* const* handles, uint32_t n, float* const* d_values, float* const* d_factors,
+		void* stream)
+{
+	if (!handles || n == 0 || !d_values || !handles[0]) return MISTA_E_INVALID_PARAM;
+	mista_handle* h0 = handles[0];
+	return guarded(h0, [&] {
+		hipStream_t s = pick_stream(h0, stream);
+		std::vector<DasBatchItem> items;
+		items.reserve(n);
+		for (uint32_t k = 0; k < n; ++k) {
+			mista_handle* h = handles[k];
+			if (!h || !d_values[k]) MISTA_THROW(MISTA_E_INVALID_PARAM, "null handle or device pointer at " << k << '.');
+			if (h->cfg.device != h0->cfg.device) MISTA_THROW(MISTA_E_INVALID_PARAM, "handles on different devices.");
+			for (uint32_t q = 0; q < k; ++q)
+				if (handles[q] == h) MISTA_THROW(MISTA_E_INVALID_PARAM, "handle " << k << " appears twice.");
+			require_grid(h);
+			if (!h->signal_ready) MISTA_THROW(MISTA_E_INVALID_STATE, "No prepared signals in handle " << k << '.');
+			h->geom.n1 = h->n1;
+			h->geom.n2 = h->n2;
+			h->geom.ny = h->ny;
+			h->geom.npix = static_cast<uint64_t>(h->n1) * h->n2;
+			h->geom.grid_version = h->grid_version;
+			upload_apod(h, s);
+			launch_weights(h->geom, h->d_rx_apod.as<const float>(), h->tx_apod_set ? h->d_tx_apod.as<const float>() : nullptr,
+					h->d_scale.as<const float>(), h->d_w.as<float>(), s);
+			items.push_back({&h->geom, &h->tuning, &h->das_scratch, h->d_tensor.get(), h->d_delay.as<const float>(),
+					h->d_w.as<const float>(), d_values[k], d_factors ? d_factors[k] : nullptr});
+		}
+		launch_das_batch(items, s);
+		for (uint32_t k = 0; k < n; ++k) handles[k]->timed = false;   // the frames share one launch
+	});
+}
+
 int mista_das_device_noalloc(mista_handle* h, float* d_value, float* d_factor, void* stream)
 {
 	if (!h) return MISTA_E_INVALID_PARAM;

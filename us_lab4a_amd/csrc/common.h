@@ -119,7 +119,8 @@ inline uint32_t pad_ntx(uint32_t ntx)
 {
 	if (ntx <= 32) return 32;
 	if (ntx <= 64) return 64;
-	return (ntx + 127) / 128 * 128;   // above 128: whole blocks of 128 (the tiled DAS runs them in turn)
+	if (ntx <= 128) return 128;
+	return (ntx + 7) / 8 * 8;   // above 128: blocks of 128 run in turn, the last one partial (whole 8-pair chunks)
 }
 
 // Bytes per complex sample of a storage kind.
@@ -199,6 +200,22 @@ constexpr size_t kNoAllocScratchBytes = static_cast<size_t>(80) << 20;
 // scratch: no device allocation; the tuned tile shape when there is one.
 void launch_das_noalloc(const FrameGeom& g, int tuned_p0, int tuned_p1, DasScratch& last, const void* d_tensor,
 		const float* d_delay, const float* d_w, float* d_value, float* d_factor, hipStream_t s);
+// One frame of a DAS batch.
+struct DasBatchItem {
+	const FrameGeom* g;
+	const DasTuning* t;
+	DasScratch* scratch;
+	const void* tensor;
+	const float* delay;
+	const float* w;
+	float* value;
+	float* factor;
+};
+// The DAS of several prepared frames: frames whose tiled launches agree in
+// storage, coherence factor, transmit block and workgroup size (and whose
+// geometry has been tuned) share one launch of the tiled kernel, up to 16 a
+// launch; the rest run one by one.  Returns the number of shared launches.
+uint32_t launch_das_batch(const std::vector<DasBatchItem>& items, hipStream_t s);
 void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, const void* d_tensor,
 		const float* d_delay, const float* d_w, float* d_value, float* d_factor, hipStream_t s);
 

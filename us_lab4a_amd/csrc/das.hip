@@ -31,10 +31,10 @@
 namespace mista {
 
 namespace das {
-extern template void launch_tile_storage<MISTA_STORAGE_F32>(int, const TileArgs&, uint32_t, uint32_t, hipStream_t);
-extern template void launch_tile_storage<MISTA_STORAGE_F16>(int, const TileArgs&, uint32_t, uint32_t, hipStream_t);
-extern template void launch_tile_storage<MISTA_STORAGE_BF16>(int, const TileArgs&, uint32_t, uint32_t, hipStream_t);
-extern template void launch_tile_storage<kStorageReal>(int, const TileArgs&, uint32_t, uint32_t, hipStream_t);
+extern template void launch_tile_storage<MISTA_STORAGE_F32>(int, const TileBatch&, uint32_t, uint32_t, hipStream_t);
+extern template void launch_tile_storage<MISTA_STORAGE_F16>(int, const TileBatch&, uint32_t, uint32_t, hipStream_t);
+extern template void launch_tile_storage<MISTA_STORAGE_BF16>(int, const TileBatch&, uint32_t, uint32_t, hipStream_t);
+extern template void launch_tile_storage<kStorageReal>(int, const TileBatch&, uint32_t, uint32_t, hipStream_t);
 }  // namespace das
 
 using namespace das;
@@ -234,7 +234,6 @@ TilePlan tile_plan(const FrameGeom& g, int p0)
 	else if (p0 == 7) { tw1 = 8; tw2 = 8; tzh = 8; }
 	tp.nt = tw1 * tw2 * tzh;
 	tp.nw = tp.nt / 64;
-	if (g.ntx_pad > 128 && tp.nt != 256) return tp;   // several transmit blocks: 256-lane tiles only
 	tp.esz = static_cast<uint32_t>(storage_bytes(g.storage));
 	const uint32_t nx = g.n1 / ny;
 	tp.tg = TileGeom{g.n1, g.n2, ny, nx, tw1, tw2, tzh, (nx + tw1 - 1) / tw1, (ny + tw2 - 1) / tw2, (g.n2 + tzh - 1) / tzh};
@@ -285,17 +284,17 @@ uint32_t build_window_tables(const FrameGeom& g, const TilePlan& tp, const float
 // budget in KiB (default: the smallest of 52 (three workgroups per CU: the
 // kernel is compiled for three waves per SIMD), 76 (two) and 150 (one) in
 // which the windows fit).  Returns false when nothing fits.
-bool launch_tiles(const FrameGeom& g, const TilePlan& tp, int p0, int p1, uint32_t S, const uint32_t* words,
-		const DasArgs& a, DasScratch& scratch, hipStream_t s)
+bool choose_chunking(const FrameGeom& g, uint32_t nt, int p1, uint32_t S, uint32_t& tc, uint32_t& nbuf)
 {
+	tc = 0;
+	nbuf = 0;
 	if (S > kMaxSlots) return false;
-	const uint32_t nt = tp.nt, nw = tp.nw;
+	const uint32_t nw = nt / 64;
 	// Pairs per chunk (tc) and chunk buffers in the LDS ring (nbuf): the most
 	// pairs in flight, (nbuf - 1) * tc, that fit the budget and the 6-bit
 	// vmcnt of the per-chunk waits.
 	const uint32_t want_tc = static_cast<uint32_t>(p1) & 0xFF, want_nb = (static_cast<uint32_t>(p1) >> 8) & 0xFF;
 	const uint32_t want_kb = (static_cast<uint32_t>(p1) >> 16) & 0xFF;
-	uint32_t tc = 0, nbuf = 0;
 	for (uint32_t kb : {52u, 76u, 150u}) {
 		if (want_kb && kb != 52u) break;
 		const size_t budget = static_cast<size_t>(want_kb ? std::min(want_kb, 158u) : kb) * 1024 -
@@ -312,13 +311,17 @@ bool launch_tiles(const FrameGeom& g, const TilePlan& tp, int p0, int p1, uint32
 		for (uint32_t c : {8u, 4u}) {
 			if (want_tc && c != want_tc) continue;
 			if (nt == 512 && c != 8) continue;
-			if (g.ntx_pad > 128 && c != 8) continue;   // several transmit blocks: 8 pairs per chunk
 			const uint32_t nb = fit(c);
 			if (nb && (nb - 1) * c > (nbuf ? (nbuf - 1) * tc : 0u)) { tc = c; nbuf = nb; }
 		}
 		if (nbuf) break;
 	}
-	if (nbuf == 0) return false;
+	return nbuf != 0;
+}
+
+TileArgs tile_args(const FrameGeom& g, const TilePlan& tp, const DasArgs& a, const uint32_t* words, uint32_t S,
+		uint32_t nbuf)
+{
 	TileArgs A;
 	A.d = a;
 	A.tg = tp.tg;
@@ -326,15 +329,55 @@ bool launch_tiles(const FrameGeom& g, const TilePlan& tp, int p0, int p1, uint32
 	A.S = S;
 	A.ntx_pad = g.ntx_pad;
 	A.nbuf = nbuf;
-	scratch.last_p0 = p0 == 0 ? (tp.tg.ny > 1 ? 4 : 1) : p0;
-	scratch.last_p1 = static_cast<int>(tc | (nbuf << 8) | (want_kb << 16));
-	switch (g.storage) {
-	case MISTA_STORAGE_F32: launch_tile_storage<MISTA_STORAGE_F32>(g.cf_kind, A, tp.ntiles, tc, s); break;
-	case MISTA_STORAGE_F16: launch_tile_storage<MISTA_STORAGE_F16>(g.cf_kind, A, tp.ntiles, tc, s); break;
-	case kStorageReal: launch_tile_storage<kStorageReal>(g.cf_kind, A, tp.ntiles, tc, s); break;
-	default: launch_tile_storage<MISTA_STORAGE_BF16>(g.cf_kind, A, tp.ntiles, tc, s); break;
+	return A;
+}
+
+// Launches the tiled kernel of the batch's storage.
+void launch_batch(int storage, int cf, const TileBatch& B, uint32_t nt, uint32_t tc, hipStream_t s)
+{
+	switch (storage) {
+	case MISTA_STORAGE_F32: launch_tile_storage<MISTA_STORAGE_F32>(cf, B, nt, tc, s); break;
+	case MISTA_STORAGE_F16: launch_tile_storage<MISTA_STORAGE_F16>(cf, B, nt, tc, s); break;
+	case kStorageReal: launch_tile_storage<kStorageReal>(cf, B, nt, tc, s); break;
+	default: launch_tile_storage<MISTA_STORAGE_BF16>(cf, B, nt, tc, s); break;
 	}
 	HIP_CHECK(hipGetLastError());
+}
+
+// One frame's tiled launch over window tables with S slots.  Returns false
+// when nothing fits.
+bool launch_tiles(const FrameGeom& g, const TilePlan& tp, int p0, int p1, uint32_t S, const uint32_t* words,
+		const DasArgs& a, DasScratch& scratch, hipStream_t s)
+{
+	uint32_t tc = 0, nbuf = 0;
+	if (!choose_chunking(g, tp.nt, p1, S, tc, nbuf)) return false;
+	TileBatch B{};
+	B.jobs[0] = tile_args(g, tp, a, words, S, nbuf);
+	B.first[0] = 0;
+	B.first[1] = tp.ntiles;
+	B.njobs = 1;
+	B.total = tp.ntiles;
+	scratch.last_p0 = p0 == 0 ? (tp.tg.ny > 1 ? 4 : 1) : p0;
+	scratch.last_p1 = static_cast<int>(tc | (nbuf << 8) | (((static_cast<uint32_t>(p1) >> 16) & 0xFF) << 16));
+	launch_batch(g.storage, g.cf_kind, B, tp.nt, tc, s);
+	return true;
+}
+
+// Window tables of tile shape p0 in the handle's scratch (built when the
+// grid, shape or offset changed: synchronises s); false when the shape does
+// not apply.
+bool ensure_tables(const FrameGeom& g, const TilePlan& tp, DasScratch& scratch, const float* d_delay, hipStream_t s)
+{
+	if (!tp.ok) return false;
+	if (scratch.bounds_version != g.grid_version || scratch.params_key != tp.key || scratch.params_offset != g.offset) {
+		float2* b = static_cast<float2*>(scratch.bounds.reserve(bounds_bytes(g, tp)));
+		uint32_t* words = static_cast<uint32_t*>(scratch.params.reserve(words_bytes(g, tp)));
+		uint32_t* d_max = static_cast<uint32_t*>(scratch.partial.reserve(256));
+		scratch.max_slots = build_window_tables(g, tp, d_delay, b, words, d_max, s);
+		scratch.bounds_version = g.grid_version;
+		scratch.params_key = tp.key;
+		scratch.params_offset = g.offset;
+	}
 	return true;
 }
 
@@ -349,16 +392,7 @@ bool launch_tiled(const FrameGeom& g, int p0, int p1, DasScratch& scratch, const
 		hipStream_t s)
 {
 	const TilePlan tp = tile_plan(g, p0);
-	if (!tp.ok) return false;
-	if (scratch.bounds_version != g.grid_version || scratch.params_key != tp.key || scratch.params_offset != g.offset) {
-		float2* b = static_cast<float2*>(scratch.bounds.reserve(bounds_bytes(g, tp)));
-		uint32_t* words = static_cast<uint32_t*>(scratch.params.reserve(words_bytes(g, tp)));
-		uint32_t* d_max = static_cast<uint32_t*>(scratch.partial.reserve(256));
-		scratch.max_slots = build_window_tables(g, tp, d_delay, b, words, d_max, s);
-		scratch.bounds_version = g.grid_version;
-		scratch.params_key = tp.key;
-		scratch.params_offset = g.offset;
-	}
+	if (!ensure_tables(g, tp, scratch, d_delay, s)) return false;
 	return launch_tiles(g, tp, p0, p1, scratch.max_slots, scratch.params.as<const uint32_t>(), a, scratch, s);
 }
 
@@ -426,7 +460,7 @@ void launch_das_noalloc(const FrameGeom& g, int tuned_p0, int tuned_p1, DasScrat
 		const float* d_delay, const float* d_w, float* d_value, float* d_factor, hipStream_t s)
 {
 	DasArgs a = das_args(g, d_tensor, d_delay, d_w, d_value, d_factor);
-	if (!((g.ntx_pad <= 128 || g.ntx_pad % 128 == 0) && static_cast<uint64_t>(g.L) * storage_bytes(g.storage) < (1ull << 31)))
+	if (!((g.ntx_pad <= 128 || g.ntx_pad % 8 == 0) && static_cast<uint64_t>(g.L) * storage_bytes(g.storage) < (1ull << 31)))
 		MISTA_THROW(MISTA_E_INVALID_PARAM, "This geometry has no tiled DAS form, which the no-allocation entry point needs.");
 	unsigned char* base = noalloc_scratch();
 	// the tuned tile shape when it fits, else the default, else the larger tiles
@@ -453,6 +487,91 @@ void launch_das_noalloc(const FrameGeom& g, int tuned_p0, int tuned_p1, DasScrat
 	MISTA_THROW(MISTA_E_INVALID_PARAM, "The signal windows of this geometry do not fit the tiled DAS kernel.");
 }
 
+// ---------------------------------------------------------------- batches
+
+uint32_t launch_das_batch(const std::vector<DasBatchItem>& items, hipStream_t s)
+{
+	struct Ready {
+		size_t i;
+		TilePlan tp;
+		int p0, p1;
+		uint64_t cls;
+	};
+	std::vector<Ready> ready;
+	std::vector<size_t> alone;
+	for (size_t i = 0; i < items.size(); ++i) {
+		const DasBatchItem& it = items[i];
+		const FrameGeom& g = *it.g;
+		int p0 = -1, p1 = 0;
+		const bool can_tile = it.t->variant != 1 && (g.ntx_pad <= 128 || g.ntx_pad % 8 == 0) &&
+				static_cast<uint64_t>(g.L) * storage_bytes(g.storage) < (1ull << 31);
+		if (can_tile && (it.t->variant == 2 || it.t->p0 || it.t->p1)) {
+			p0 = it.t->p0;
+			p1 = it.t->p1;
+		} else if (can_tile && it.scratch->tuned_p0 >= 0) {
+			p0 = it.scratch->tuned_p0;
+			p1 = it.scratch->tuned_p1;
+		}
+		TilePlan tp;
+		if (p0 >= 0) tp = tile_plan(g, p0);
+		if (p0 < 0 || !ensure_tables(g, tp, *it.scratch, it.delay, s) || it.scratch->max_slots > kMaxSlots) {
+			alone.push_back(i);   // untuned geometry (it tunes alone), gather kernel, or no tiled form
+			continue;
+		}
+		const uint32_t ncls = g.ntx_pad > 128 ? 256u : g.ntx_pad;   // transmit block instantiation
+		const uint64_t cls = static_cast<uint64_t>(g.storage) | (static_cast<uint64_t>(g.cf_kind) << 8) |
+				(static_cast<uint64_t>(ncls) << 16) | (static_cast<uint64_t>(tp.nt) << 32);
+		ready.push_back({i, tp, p0, p1, cls});
+	}
+	uint32_t shared = 0;
+	std::vector<bool> done(ready.size(), false);
+	for (size_t a0 = 0; a0 < ready.size(); ++a0) {
+		if (done[a0]) continue;
+		// the next group: frames of a0's class, in order, at most kMaxBatch
+		std::vector<size_t> grp;
+		for (size_t b = a0; b < ready.size() && grp.size() < kMaxBatch; ++b)
+			if (!done[b] && ready[b].cls == ready[a0].cls) grp.push_back(b);
+		uint32_t S = 0;
+		bool same_p1 = true;
+		for (size_t b : grp) {
+			S = std::max(S, items[ready[b].i].scratch->max_slots);
+			same_p1 = same_p1 && ready[b].p1 == ready[grp[0]].p1;
+		}
+		const FrameGeom& g0 = *items[ready[grp[0]].i].g;
+		uint32_t tc = 0, nbuf = 0;
+		const uint32_t nt = ready[grp[0]].tp.nt;
+		bool fits = choose_chunking(g0, nt, same_p1 ? ready[grp[0]].p1 : 0, S, tc, nbuf);
+		if (!fits) fits = choose_chunking(g0, nt, 0, S, tc, nbuf);
+		for (size_t b : grp) done[b] = true;
+		if (!fits || grp.size() == 1) {
+			for (size_t b : grp) alone.push_back(ready[b].i);
+			continue;
+		}
+		TileBatch B{};
+		B.njobs = static_cast<uint32_t>(grp.size());
+		uint32_t total = 0;
+		for (uint32_t k = 0; k < B.njobs; ++k) {
+			const Ready& r = ready[grp[k]];
+			const DasBatchItem& it = items[r.i];
+			const DasArgs a = das_args(*it.g, it.tensor, it.delay, it.w, it.value, it.factor);
+			B.jobs[k] = tile_args(*it.g, r.tp, a, it.scratch->params.as<const uint32_t>(), S, nbuf);
+			B.first[k] = total;
+			total += r.tp.ntiles;
+			it.scratch->last_p0 = r.p0 == 0 ? (r.tp.tg.ny > 1 ? 4 : 1) : r.p0;
+			it.scratch->last_p1 = static_cast<int>(tc | (nbuf << 8));
+		}
+		B.first[B.njobs] = total;
+		B.total = total;
+		launch_batch(g0.storage, g0.cf_kind, B, nt, tc, s);
+		++shared;
+	}
+	for (size_t i : alone) {
+		const DasBatchItem& it = items[i];
+		launch_das(*it.g, *it.t, *it.scratch, it.tensor, it.delay, it.w, it.value, it.factor, s);
+	}
+	return shared;
+}
+
 // Tail padding of the analytic tensor (bytes): a staged window is copied as
 // whole 1 KiB slots from its first entry, so the last channel's copies may
 // read up to kMaxSlots KiB past its last sample.
@@ -464,7 +583,7 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 	const DasArgs a = das_args(g, d_tensor, d_delay, d_w, d_value, d_factor);
 
 	// window words hold ws * ESZ in 31 bits (L <= 2^20 samples)
-	const bool can_tile = (t.variant != 1) && (g.ntx_pad <= 128 || g.ntx_pad % 128 == 0) &&
+	const bool can_tile = (t.variant != 1) && (g.ntx_pad <= 128 || g.ntx_pad % 8 == 0) &&
 			static_cast<uint64_t>(g.L) * storage_bytes(g.storage) < (1ull << 31);
 	if (can_tile) {
 		if (t.variant == 2 || t.p0 || t.p1) {
@@ -527,12 +646,19 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 					cands.push_back(0);
 					for (const int p1 : cands) {
 						if (bp0 >= 0 && best > kTuneLimitMs) break;
-						HIP_CHECK(hipEventRecord(e0, s));
-						const bool run = launch_tiled(g, p0, p1, scratch, a, d_delay, s);
-						HIP_CHECK(hipEventRecord(e1, s));
-						HIP_CHECK(hipEventSynchronize(e1));
+						// the better of two runs (the first may find cold caches)
 						float ms = 0.0f;
-						HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+						bool run = false;
+						for (int rep = 0; rep < 2; ++rep) {
+							HIP_CHECK(hipEventRecord(e0, s));
+							run = launch_tiled(g, p0, p1, scratch, a, d_delay, s);
+							HIP_CHECK(hipEventRecord(e1, s));
+							HIP_CHECK(hipEventSynchronize(e1));
+							float t = 0.0f;
+							HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+							ms = rep == 0 ? t : std::min(ms, t);
+							if (!run || t > kTuneLimitMs) break;
+						}
 						if (run && (bp0 < 0 || ms < best)) {
 							best = ms;
 							bp0 = p0;

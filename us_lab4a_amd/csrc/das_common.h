@@ -310,6 +310,17 @@ struct TileArgs {
 	uint32_t nbuf;            // chunk buffers in the LDS ring (>= 2)
 };
 
+// The tiles of up to kMaxBatch frames (possibly of different shapes) in one
+// launch: frame j's tiles are global tiles first[j] .. first[j+1]-1.  Every
+// frame shares the kernel instantiation (storage, coherence factor, transmit
+// block, chunk size, workgroup size) and the window size S and ring depth.
+constexpr uint32_t kMaxBatch = 16;
+struct TileBatch {
+	TileArgs jobs[kMaxBatch];
+	uint32_t first[kMaxBatch + 1];
+	uint32_t njobs, total;
+};
+
 constexpr uint32_t kTileThreads = 256;
 // Zeroed bytes after the analytic tensor: a window is staged as whole 1 KiB
 // slots from its first entry, so the last channel's copies may read up to
@@ -436,9 +447,7 @@ __device__ __forceinline__ void lds_read_pairs(const uint32_t* addr, typename Ld
 template<int NTHREADS> constexpr int tile_waves_per_eu() { return NTHREADS >= 512 ? 2 : 3; }
 
 template<int STORAGE, int CF, int NTXMAX, int TC, int NTHREADS, bool MULTI>
-__global__ void __launch_bounds__(NTHREADS)
-__attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<NTHREADS>(), tile_waves_per_eu<NTHREADS>())))
-das_tile_kernel(const TileArgs A)
+__device__ __forceinline__ void das_tile(const TileArgs& A, const uint32_t tile, unsigned char* smem)
 {
 	using E = typename Entry<STORAGE>::T;
 	constexpr uint32_t NT = NTHREADS;       // one tile point per lane
@@ -453,14 +462,8 @@ das_tile_kernel(const TileArgs A)
 	// registers, else one at a time (128 registers hold transmit delays)
 	constexpr int BATCH = NTHREADS >= 512 ? TC : (NTXMAX <= 64 ? 4 : 1);
 	static_assert(BATCH == 1 || TC % BATCH == 0, "whole read groups per chunk");
-	extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
 
 	const DasArgs& a = A.d;
-	// XCD-aware order: workgroups are dealt round-robin to the 8 XCDs
-	// (block b runs on XCD b % 8), so give XCD x one contiguous run of tiles;
-	// tiles running at the same time on one XCD then share windows in its L2.
-	const uint32_t ntiles = gridDim.x, q8 = ntiles / 8, rm8 = ntiles % 8;
-	const uint32_t xcd = blockIdx.x % 8, tile = xcd * q8 + min(xcd, rm8) + blockIdx.x / 8;
 	const uint32_t tid = threadIdx.x, lane = tid % 64;
 	const uint32_t wave = __builtin_amdgcn_readfirstlane(tid / 64);   // uniform: scalar copy loops
 	uint32_t col, row;
@@ -470,7 +473,9 @@ das_tile_kernel(const TileArgs A)
 	const uint64_t p = static_cast<uint64_t>(col) * A.tg.n2 + row;
 	const uint32_t L = a.L, ntx = a.ntx, nrx = a.nrx;
 	// transmits run in blocks of NTXMAX (MULTI: several, at NTXMAX = 128)
-	const uint32_t nblk = MULTI ? A.ntx_pad / NTXMAX : 1u, CPR = nblk * CPB;   // chunks per rx
+	// (MULTI: several, the last one holding the rest in whole chunks)
+	const uint32_t nblk = MULTI ? (A.ntx_pad + NTXMAX - 1) / NTXMAX : 1u;
+	const uint32_t CPR = MULTI ? A.ntx_pad / TC : CPB;   // chunks per rx
 	const float lim = static_cast<float>(L - 1u);
 	const uint32_t S = A.S, NB = A.nbuf;
 	const uint32_t wcap = S * SLOT;
@@ -540,12 +545,6 @@ das_tile_kernel(const TileArgs A)
 			is_drx += npix;
 		}
 	};
-	// Windows of pairs with nothing to stage are read (with weight 0): start
-	// from zeros so that no stale bit pattern can be Inf/NaN.
-	for (uint32_t i = tid; i < NB * buf_bytes / 16; i += NT)
-		reinterpret_cast<uint4*>(smem)[i] = make_uint4(0u, 0u, 0u, 0u);
-	__syncthreads();
-
 	const uint32_t pre = min(NB - 1, nchunks);
 	for (uint32_t k = 0; k < pre; ++k) {
 		load_issue_words();
@@ -575,6 +574,7 @@ das_tile_kernel(const TileArgs A)
 			if (MULTI) load_dtx(blk);   // waited for at its first use
 #pragma clang loop unroll(full)
 			for (int c = 0; c < static_cast<int>(CPB); ++c, ++i) {
+				if (MULTI && blk * CPB + static_cast<uint32_t>(c) >= CPR) break;   // the last block's end
 				// this chunk's words and weights (loaded one chunk ahead)
 				uint32_t w_[TC];
 				float wt[TC];
@@ -681,6 +681,56 @@ das_tile_kernel(const TileArgs A)
 	if (valid) st.write(a, p);
 }
 
+// A workgroup per tile.  XCD-aware order: workgroups are dealt round-robin
+// to the 8 XCDs (block b runs on XCD b % 8), so XCD x gets one contiguous run
+// of the tiles; tiles running at the same time on one XCD then share windows
+// in its L2.
+__device__ __forceinline__ uint32_t xcd_tile()
+{
+	const uint32_t T = gridDim.x, q8 = T / 8, rm8 = T % 8, xcd = blockIdx.x % 8;
+	return xcd * q8 + min(xcd, rm8) + blockIdx.x / 8;
+}
+
+// Windows of pairs with nothing to stage are read (with weight 0): the ring
+// starts from zeros so that no stale bit pattern can be Inf/NaN.
+template<int TC, int NTHREADS>
+__device__ __forceinline__ void zero_ring(unsigned char* smem, uint32_t nbuf, uint32_t S)
+{
+	const uint32_t ring = nbuf * (TC * S * 1024u + NTHREADS * 4u);
+	for (uint32_t i = threadIdx.x; i < ring / 16; i += NTHREADS)
+		reinterpret_cast<uint4*>(smem)[i] = make_uint4(0u, 0u, 0u, 0u);
+	__syncthreads();
+}
+
+// One frame: its arguments arrive as the kernel's own, which the compiler
+// keeps in scalar registers for the whole tile.
+template<int STORAGE, int CF, int NTXMAX, int TC, int NTHREADS, bool MULTI>
+__global__ void __launch_bounds__(NTHREADS)
+__attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<NTHREADS>(), tile_waves_per_eu<NTHREADS>())))
+das_tile_kernel(const TileArgs A)
+{
+	extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+	zero_ring<TC, NTHREADS>(smem, A.nbuf, A.S);
+	das_tile<STORAGE, CF, NTXMAX, TC, NTHREADS, MULTI>(A, xcd_tile(), smem);
+}
+
+// A batch of frames in one launch, their tiles concatenated: the hardware
+// dispatcher starts each workgroup as a CU frees up, so frames of uneven cost
+// balance by themselves.  (A separate kernel: a frame's arguments picked from
+// the table by a run-time index are reloaded from memory at their uses.)
+template<int STORAGE, int CF, int NTXMAX, int TC, int NTHREADS, bool MULTI>
+__global__ void __launch_bounds__(NTHREADS)
+__attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<NTHREADS>(), tile_waves_per_eu<NTHREADS>())))
+das_tile_batch_kernel(const TileBatch B)
+{
+	extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+	zero_ring<TC, NTHREADS>(smem, B.jobs[0].nbuf, B.jobs[0].S);
+	const uint32_t g = xcd_tile();
+	uint32_t j = 0;
+	while (j + 1 < B.njobs && B.first[j + 1] <= g) ++j;
+	das_tile<STORAGE, CF, NTXMAX, TC, NTHREADS, MULTI>(B.jobs[j], g - B.first[j], smem);
+}
+
 // LDS bytes of one chunk buffer: TC windows of S 1 KiB slots + the rx delays.
 inline size_t tile_buffer_bytes(uint32_t tc, uint32_t S, uint32_t nt)
 {
@@ -688,42 +738,55 @@ inline size_t tile_buffer_bytes(uint32_t tc, uint32_t S, uint32_t nt)
 }
 
 template<int STORAGE, int CF, int NTXMAX>
-void launch_tile_t(const TileArgs& A, uint32_t ntiles, uint32_t tc, hipStream_t s)
+void launch_tile_t(const TileBatch& B, uint32_t nt, uint32_t tc, hipStream_t s)
 {
-	const uint32_t nt = A.tg.tw1 * A.tg.tw2 * A.tg.tzh;   // 256 or 512 (512: tc = 8)
+	const TileArgs& A = B.jobs[0];   // S, nbuf and the transmit padding are shared by the batch
 	const size_t lds = A.nbuf * tile_buffer_bytes(tc, A.S, nt) + tile_stats_bytes(CF, nt);
-	auto go = [&](auto kern, uint32_t threads) {
-		HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-				static_cast<int>(lds)));
-		hipLaunchKernelGGL(kern, dim3(ntiles), dim3(threads), lds, s, A);
+	const uint32_t grid = B.total;   // a workgroup per tile
+	auto go1 = [&](auto kern, auto kern_batch, uint32_t threads) {
+		if (B.njobs == 1) {
+			HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+					static_cast<int>(lds)));
+			hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, s, A);
+		} else {
+			HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern_batch),
+					hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+			hipLaunchKernelGGL(kern_batch, dim3(grid), dim3(threads), lds, s, B);
+		}
 	};
+#define MISTA_GO(tcv, ntv, multi) \
+	go1(das_tile_kernel<STORAGE, CF, NTXMAX, tcv, ntv, multi>, das_tile_batch_kernel<STORAGE, CF, NTXMAX, tcv, ntv, multi>, ntv)
 	if constexpr (NTXMAX == 128) {
-		if (A.ntx_pad > NTXMAX) {   // several transmit blocks: tc = 8, nt = 256
-			go(das_tile_kernel<STORAGE, CF, NTXMAX, 8, 256, true>, 256);
+		if (A.ntx_pad > NTXMAX) {   // several transmit blocks
+			if (nt == 512) MISTA_GO(8, 512, true);
+			else if (tc == 8) MISTA_GO(8, 256, true);
+			else MISTA_GO(4, 256, true);
 			return;
 		}
 	}
-	if (nt == 512) go(das_tile_kernel<STORAGE, CF, NTXMAX, 8, 512, false>, 512);
-	else if (tc == 8) go(das_tile_kernel<STORAGE, CF, NTXMAX, 8, 256, false>, 256);
-	else go(das_tile_kernel<STORAGE, CF, NTXMAX, 4, 256, false>, 256);
+	if (nt == 512) MISTA_GO(8, 512, false);
+	else if (tc == 8) MISTA_GO(8, 256, false);
+	else MISTA_GO(4, 256, false);
+#undef MISTA_GO
 }
 
 template<int STORAGE, int CF>
-void launch_tile_n(const TileArgs& A, uint32_t ntiles, uint32_t tc, hipStream_t s)
+void launch_tile_n(const TileBatch& B, uint32_t nt, uint32_t tc, hipStream_t s)
 {
-	if (A.ntx_pad == 32) launch_tile_t<STORAGE, CF, 32>(A, ntiles, tc, s);
-	else if (A.ntx_pad == 64) launch_tile_t<STORAGE, CF, 64>(A, ntiles, tc, s);
-	else launch_tile_t<STORAGE, CF, 128>(A, ntiles, tc, s);
+	const uint32_t ntx_pad = B.jobs[0].ntx_pad;
+	if (ntx_pad == 32) launch_tile_t<STORAGE, CF, 32>(B, nt, tc, s);
+	else if (ntx_pad == 64) launch_tile_t<STORAGE, CF, 64>(B, nt, tc, s);
+	else launch_tile_t<STORAGE, CF, 128>(B, nt, tc, s);
 }
 
 // Tiled launch for one storage kind (instantiated in das_tile_<storage>.hip).
 template<int STORAGE>
-void launch_tile_storage(int cf, const TileArgs& A, uint32_t ntiles, uint32_t tc, hipStream_t s)
+void launch_tile_storage(int cf, const TileBatch& B, uint32_t nt, uint32_t tc, hipStream_t s)
 {
 	switch (cf) {
-	case MISTA_CF_NONE: launch_tile_n<STORAGE, MISTA_CF_NONE>(A, ntiles, tc, s); break;
-	case MISTA_CF_PCF: launch_tile_n<STORAGE, MISTA_CF_PCF>(A, ntiles, tc, s); break;
-	default: launch_tile_n<STORAGE, MISTA_CF_SCF>(A, ntiles, tc, s); break;
+	case MISTA_CF_NONE: launch_tile_n<STORAGE, MISTA_CF_NONE>(B, nt, tc, s); break;
+	case MISTA_CF_PCF: launch_tile_n<STORAGE, MISTA_CF_PCF>(B, nt, tc, s); break;
+	default: launch_tile_n<STORAGE, MISTA_CF_SCF>(B, nt, tc, s); break;
 	}
 }
 

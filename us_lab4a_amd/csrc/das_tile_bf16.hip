@@ -3,6 +3,6 @@
 
 namespace mista {
 namespace das {
-template void launch_tile_storage<MISTA_STORAGE_BF16>(int, const TileArgs&, uint32_t, uint32_t, hipStream_t);
+template void launch_tile_storage<MISTA_STORAGE_BF16>(int, const TileBatch&, uint32_t, uint32_t, hipStream_t);
 }  // namespace das
 }  // namespace mista

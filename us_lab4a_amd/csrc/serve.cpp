@@ -71,7 +71,7 @@ struct Job {
 	uint64_t key = 0;
 	size_t estimate = 0;
 	Clock::time_point t_submit, t_admit, t_done;
-	hipEvent_t ev_start = nullptr, ev_end = nullptr;
+	hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_prep = nullptr;
 	int slot = -1;
 	uint32_t skips = 0;      // admissions that passed this job at the queue's head
 	bool launched = false;   // its work is on the device (ev_end recorded)
@@ -159,6 +159,8 @@ struct mista_engine {
 	bool stop = false;
 	std::string last_error;
 	std::thread worker, builder;
+	hipStream_t das_stream = nullptr;   // DAS of warm jobs, batched per admission round
+	std::vector<Job*> pending;          // warm jobs prepared, DAS not yet enqueued
 
 	void run();
 	void build_loop();
@@ -168,6 +170,8 @@ struct mista_engine {
 	int new_slot(uint64_t key, size_t bytes);
 	void drop_slot(Slot& s);
 	void launch(Job* j, Slot& s, mista_handle* h);
+	void prep_job(Job* j, Slot& s, mista_handle* h);
+	void flush(std::unique_lock<std::mutex>& lk);
 	void fail(Job* j, int status, const std::string& err);
 	void finish(Job* j);
 	void poll_running();
@@ -183,7 +187,8 @@ void mista_engine::finish(Job* j)
 	}
 	if (j->ev_start) (void)hipEventDestroy(j->ev_start);
 	if (j->ev_end) (void)hipEventDestroy(j->ev_end);
-	j->ev_start = j->ev_end = nullptr;
+	if (j->ev_prep) (void)hipEventDestroy(j->ev_prep);
+	j->ev_start = j->ev_end = j->ev_prep = nullptr;
 	if (j->slot >= 0) {
 		Slot& s = slots[j->slot];
 		s.inflight -= 1;
@@ -257,6 +262,88 @@ void mista_engine::launch(Job* j, Slot& s, mista_handle* h)
 	HIP_CHECK(hipEventRecord(j->ev_end, st));
 }
 
+// Enqueues a warm job's preparation on its slot's stream (host echoes staged
+// in the slot's buffer); its DAS follows in the round's batch (flush).
+void mista_engine::prep_job(Job* j, Slot& s, mista_handle* h)
+{
+	const mista_job& q = j->spec;
+	hipStream_t st = handle_stream(h);
+	const uint64_t ntx = q.config.last_tx_elem - q.config.first_tx_elem + 1ull;
+	const uint64_t nraw = ntx * q.config.num_elements * q.num_samples;
+	HIP_CHECK(hipEventCreate(&j->ev_start));
+	HIP_CHECK(hipEventCreate(&j->ev_end));
+	HIP_CHECK(hipEventCreate(&j->ev_prep));
+	HIP_CHECK(hipEventRecord(j->ev_start, st));
+	const float* d_echoes = q.echoes;
+	if (!q.echoes_on_device) {
+		float* d = static_cast<float*>(s.d_echoes.reserve(nraw * sizeof(float)));
+		HIP_CHECK(hipMemcpyAsync(d, q.echoes, nraw * sizeof(float), hipMemcpyHostToDevice, st));
+		d_echoes = d;
+	}
+	const int rc = mista_prepare_signals_device(h, d_echoes, q.num_samples, nullptr);
+	if (rc != MISTA_OK) MISTA_THROW(rc, mista_last_error(h));
+	HIP_CHECK(hipEventRecord(j->ev_prep, st));
+}
+
+// The DAS of the round's prepared warm jobs: one batched call on the DAS
+// stream after their preparations (compatible frames share a launch); each
+// job's end is recorded there, and its slot's stream waits for it, so the
+// handle's next preparation cannot overwrite the tensor too early.  Called
+// with the lock held; released while enqueueing.
+void mista_engine::flush(std::unique_lock<std::mutex>& lk)
+{
+	if (pending.empty()) return;
+	std::vector<Job*> js;
+	js.swap(pending);
+	lk.unlock();
+	int status = MISTA_OK;
+	std::string err;
+	try {
+		std::vector<mista_handle*> hs;
+		std::vector<float*> vs, fs;
+		for (Job* j : js) {
+			Slot& s = slots[j->slot];
+			const mista_job& q = j->spec;
+			const uint64_t npix = static_cast<uint64_t>(q.n1) * q.n2;
+			HIP_CHECK(hipStreamWaitEvent(das_stream, j->ev_prep, 0));
+			hs.push_back(s.h);
+			if (q.outputs_on_device) {
+				vs.push_back(q.value);
+				fs.push_back(q.factor);
+			} else {
+				vs.push_back(static_cast<float*>(s.d_value.reserve(npix * sizeof(float))));
+				fs.push_back(q.factor ? static_cast<float*>(s.d_factor.reserve(npix * sizeof(float))) : nullptr);
+			}
+		}
+		const int rc = mista_das_batch_device(hs.data(), static_cast<uint32_t>(hs.size()), vs.data(), fs.data(), das_stream);
+		if (rc != MISTA_OK) MISTA_THROW(rc, mista_last_error(hs[0]));
+		for (size_t k = 0; k < js.size(); ++k) {
+			Job* j = js[k];
+			const mista_job& q = j->spec;
+			const uint64_t npix = static_cast<uint64_t>(q.n1) * q.n2;
+			int p1 = 0;
+			(void)mista_last_kernel(hs[k], &j->das_tile, &p1);
+			if (!q.outputs_on_device) {
+				HIP_CHECK(hipMemcpyAsync(q.value, vs[k], npix * sizeof(float), hipMemcpyDeviceToHost, das_stream));
+				if (q.factor) HIP_CHECK(hipMemcpyAsync(q.factor, fs[k], npix * sizeof(float), hipMemcpyDeviceToHost, das_stream));
+			}
+			HIP_CHECK(hipEventRecord(j->ev_end, das_stream));
+			HIP_CHECK(hipStreamWaitEvent(handle_stream(hs[k]), j->ev_end, 0));
+		}
+	} catch (const Error& e) {
+		status = e.code;
+		err = e.what();
+	} catch (const std::exception& e) {
+		status = MISTA_E_INVALID_STATE;
+		err = e.what();
+	}
+	lk.lock();
+	for (Job* j : js) {
+		if (status != MISTA_OK) fail(j, status, err);
+		else j->launched = true;
+	}
+}
+
 // Completes a job that could not run (called with the lock held).
 void mista_engine::fail(Job* j, int status, const std::string& err)
 {
@@ -264,7 +351,8 @@ void mista_engine::fail(Job* j, int status, const std::string& err)
 	j->status = status;
 	if (j->ev_start) (void)hipEventDestroy(j->ev_start);
 	if (j->ev_end) (void)hipEventDestroy(j->ev_end);
-	j->ev_start = j->ev_end = nullptr;
+	if (j->ev_prep) (void)hipEventDestroy(j->ev_prep);
+	j->ev_start = j->ev_end = j->ev_prep = nullptr;
 	auto it = std::find(running.begin(), running.end(), j);
 	if (it != running.end()) running.erase(it);
 	if (j->slot >= 0) slots[j->slot].inflight -= 1;
@@ -392,11 +480,18 @@ bool mista_engine::admit_one(std::unique_lock<std::mutex>& lk)
 			cv_build.notify_one();
 			return true;
 		}
+		// a second job on a handle prepares only after the first one's DAS
+		// is enqueued (its slot stream then waits for that DAS)
+		for (Job* o : pending)
+			if (o->slot == si) {
+				flush(lk);
+				break;
+			}
 		lk.unlock();
 		int status = MISTA_OK;
 		std::string err;
 		try {
-			launch(j, s, s.h);
+			prep_job(j, s, s.h);
 		} catch (const Error& e) {
 			status = e.code;
 			err = e.what();
@@ -406,7 +501,7 @@ bool mista_engine::admit_one(std::unique_lock<std::mutex>& lk)
 		}
 		lk.lock();
 		if (status != MISTA_OK) fail(j, status, err);
-		else j->launched = true;
+		else pending.push_back(j);
 		return true;
 	}
 	return false;
@@ -420,6 +515,7 @@ void mista_engine::run()
 		poll_running();
 		bool progressed = false;
 		while (admit_one(lk)) progressed = true;
+		flush(lk);
 		if (stop && queue.empty() && running.empty() && n_building == 0) break;
 		if (progressed) continue;
 		if (running.empty() && n_building == 0) {
@@ -573,6 +669,7 @@ int mista_engine_create(const mista_engine_config* config, mista_engine** out)
 			HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
 			e->budget = free_b / 10 * 8;
 		}
+		HIP_CHECK(hipStreamCreateWithFlags(&e->das_stream, hipStreamNonBlocking));
 		mista_engine* p = e.get();
 		e->worker = std::thread([p] { p->run(); });
 		e->builder = std::thread([p] { p->build_loop(); });
@@ -602,6 +699,7 @@ int mista_engine_destroy(mista_engine* e)
 	(void)hipSetDevice(e->cfg.device);
 	for (auto& s : e->slots)
 		if (s.h) (void)mista_destroy(s.h);
+	if (e->das_stream) (void)hipStreamDestroy(e->das_stream);
 	for (auto& kv : e->jobs) delete kv.second;
 	delete e;
 	return MISTA_OK;

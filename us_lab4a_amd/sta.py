@@ -273,6 +273,19 @@ class HipSTAProcessor:
         return p.value, d.value
 
 
+def das_batch_device(processors, d_values, d_factors=None, stream=None):
+    """DAS of several prepared processors: compatible tuned frames share one
+    launch of the tiled kernel (mista_das_batch_device); each writes the
+    images it writes alone."""
+    n = len(processors)
+    if n == 0 or len(d_values) != n or (d_factors is not None and len(d_factors) != n):
+        raise _lib.InvalidParameterError(_lib.MISTA_E_INVALID_PARAM, "one value (and factor) buffer per processor")
+    hs = (ctypes.c_void_p * n)(*[p._h for p in processors])
+    vs = (ctypes.c_void_p * n)(*[_ptr(v) for v in d_values])
+    fs = None if d_factors is None else (ctypes.c_void_p * n)(*[_ptr(f) for f in d_factors])
+    check(lib().mista_das_batch_device(hs, n, vs, fs, _stream_ptr(stream)), processors[0]._h)
+
+
 @dataclass
 class SA3DConfiguration:
     """Fields of SA3DConfiguration<float> on the path (SA3DConfiguration.h:37-58):
