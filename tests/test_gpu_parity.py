@@ -325,30 +325,38 @@ def test_fused_reduced_storage_is_scaled_fp32(shape, storage):
     assert np.array_equal(out[storage], expect)
 
 
-@pytest.mark.parametrize("cf", ["none", "phase_coherence_factor", "sign_coherence_factor"])
-def test_tiled_kernel_over_128_transmits(cf):
-    """192 elements (transmits padded to two blocks of 128, the second one
-    partly empty): the tiled kernel runs the blocks in turn.  With a
-    coherence factor it keeps the gather kernel's per-rx order and is
-    bit-identical to it (without one the gather kernel sums transmit-outer,
-    like the reference's no-CF loop, so the two agree to the tolerance);
-    every tiled shape gives the same image; the frame matches the oracle."""
+@pytest.mark.parametrize("elements,cf", [(192, "none"), (192, "phase_coherence_factor"), (192, "sign_coherence_factor"),
+                                         (200, "phase_coherence_factor"), (136, "sign_coherence_factor"),
+                                         (300, "phase_coherence_factor")])
+def test_tiled_kernel_over_128_transmits(elements, cf):
+    """More than 128 transmits: the tiled kernel runs them in equal blocks of
+    whole 8-pair chunks (192: two of 96; 200: two of 104, the second with 8
+    padding transmits; 136: two of 72; 300: three of 104), block after block
+    over all receive elements, each receive element's partial sum kept in
+    HBM between blocks.  With a coherence factor the sum per receive element
+    runs over the transmits in order, the gather kernel's fma chain, so the
+    two are bit-identical (without one the gather kernel sums
+    transmit-outer, like the reference's no-CF loop, and the two agree to
+    the tolerance).  Both 512-lane tile shapes give the same image; a
+    256-lane tile is refused and the gather kernel runs; the frame matches
+    the oracle."""
     import torch
-    wl = workloads.tiny(num_elements=192, num_samples=2048, upsampling=2, cf=cf, n_x=24, n_z=32)
+    wl = workloads.tiny(num_elements=elements, num_samples=2048, upsampling=2, cf=cf, n_x=24, n_z=32)
     echoes = wl.echoes_cpu()
-    say("over 128 transmits", cf)
+    say("over 128 transmits", elements, cf)
     p = make_proc(wl)
     p.process_echoes(echoes)
     n = wl.n_x * wl.n_z
     out = {}
-    for variant, p0 in ((1, 0), (2, 1)):
+    for variant, p0 in ((1, 0), (2, 5), (2, 6), (2, 1)):
         p.set_kernel(variant, p0, 0)
         dv = torch.empty(n, dtype=torch.float32, device="cuda")
         df = torch.empty(n, dtype=torch.float32, device="cuda")
         p.das_device(dv, df)
         torch.cuda.synchronize()
-        out[variant] = (dv.cpu().numpy().reshape(wl.n_x, wl.n_z), df.cpu().numpy().reshape(wl.n_x, wl.n_z))
-        assert (p.last_kernel()[0] >= 0) == (variant == 2)   # the tiled kernel ran, not the fallback
+        out[p0] = (dv.cpu().numpy().reshape(wl.n_x, wl.n_z), df.cpu().numpy().reshape(wl.n_x, wl.n_z))
+        # 512-lane shapes run tiled; the 256-lane one falls back to the gather kernel
+        assert (p.last_kernel()[0] >= 0) == (p0 in (5, 6)), (p0, p.last_kernel())
     p.set_kernel(0)
     v_a, f_a = p.process_echoes(echoes)
     tuned = p.tuned_kernel()
@@ -356,19 +364,21 @@ def test_tiled_kernel_over_128_transmits(cf):
     d_gpu = p.download_delays()
     p.close()
     say("autotuner chose", tuned)
-    assert tuned[0] >= 0
-    assert np.array_equal(v_a.reshape(wl.n_x, wl.n_z), out[2][0])
+    assert tuned[0] in (5, 6)
+    assert np.array_equal(out[5][0], out[6][0]) and np.array_equal(out[5][1], out[6][1])
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(v_a.reshape(wl.n_x, wl.n_z), out[5][0])
     if cf == "none":
-        assert peak_err(out[2][0], out[1][0]) <= TOL
+        assert peak_err(out[5][0], out[0][0]) <= TOL
     else:
-        assert np.array_equal(out[1][0], out[2][0]) and np.array_equal(out[1][1], out[2][1])
+        assert np.array_equal(out[0][0], out[5][0]) and np.array_equal(out[0][1], out[5][1])
     ocfg = oracle.cfg_for(wl)
     a32 = oracle.prepare(ocfg, echoes, np.float32)
     xz = imaging.grid_xz(wl.grid())
-    check_table_oracle(out[2][0], out[2][1], ocfg, a_gpu, d_gpu, wl.rx_apodization(), wl.tx_apodization(),
-                       f"over 128 transmits {cf}")
-    check_reference_oracle(out[2][0], out[2][1], ocfg, a32, xz, d_gpu, wl.rx_apodization(), wl.tx_apodization(),
-                           f"over 128 transmits {cf}")
+    check_table_oracle(out[5][0], out[5][1], ocfg, a_gpu, d_gpu, wl.rx_apodization(), wl.tx_apodization(),
+                       f"over 128 transmits {elements} {cf}")
+    check_reference_oracle(out[5][0], out[5][1], ocfg, a32, xz, d_gpu, wl.rx_apodization(), wl.tx_apodization(),
+                           f"over 128 transmits {elements} {cf}")
 
 
 @pytest.fixture(scope="module")

@@ -120,7 +120,10 @@ inline uint32_t pad_ntx(uint32_t ntx)
 	if (ntx <= 32) return 32;
 	if (ntx <= 64) return 64;
 	if (ntx <= 128) return 128;
-	return (ntx + 7) / 8 * 8;   // above 128: blocks of 128 run in turn, the last one partial (whole 8-pair chunks)
+	// above 128: ceil(ntx / 128) equal blocks of whole 8-pair chunks, run in turn
+	const uint32_t nblk = (ntx + 127) / 128;
+	const uint32_t bt = ((ntx + nblk - 1) / nblk + 7) / 8 * 8;
+	return nblk * bt;
 }
 
 // Bytes per complex sample of a storage kind.
@@ -184,6 +187,7 @@ struct DasScratch {
 	DeviceBuffer partial;           // small device scalars (max window slots)
 	DeviceBuffer bounds;            // per-tile element delay bounds
 	DeviceBuffer params;            // per (tile, rx, tx) window words
+	DeviceBuffer partials;          // per-rx sums between transmit blocks (more than 128 transmits)
 	uint64_t bounds_version = ~0ull;
 	uint64_t params_key = 0;
 	float params_offset = 0.0f;

@@ -234,6 +234,10 @@ TilePlan tile_plan(const FrameGeom& g, int p0)
 	else if (p0 == 7) { tw1 = 8; tw2 = 8; tzh = 8; }
 	tp.nt = tw1 * tw2 * tzh;
 	tp.nw = tp.nt / 64;
+	// More than 128 transmits: 512-lane tiles only.  A lane holds a block's
+	// 128 transmit delays in registers, which leaves a 256-lane tile (three
+	// waves per SIMD, 168 registers) spilling; 512-lane tiles have 256.
+	if (g.ntx_pad > 128 && tp.nt != 512) return tp;
 	tp.esz = static_cast<uint32_t>(storage_bytes(g.storage));
 	const uint32_t nx = g.n1 / ny;
 	tp.tg = TileGeom{g.n1, g.n2, ny, nx, tw1, tw2, tzh, (nx + tw1 - 1) / tw1, (ny + tw2 - 1) / tw2, (g.n2 + tzh - 1) / tzh};
@@ -396,11 +400,25 @@ bool launch_tiled(const FrameGeom& g, int p0, int p1, DasScratch& scratch, const
 	return launch_tiles(g, tp, p0, p1, scratch.max_slots, scratch.params.as<const uint32_t>(), a, scratch, s);
 }
 
+// Bytes of the per-rx partial sums a frame with several transmit blocks
+// keeps between blocks (tiled kernel), 0 with one block.
+size_t partial_bytes(const FrameGeom& g)
+{
+	return g.ntx_pad > 128 ? static_cast<size_t>(g.nrx) * g.npix * sizeof(float2) : 0;
+}
+
+float2* scratch_partial(const FrameGeom& g, DasScratch& scratch)
+{
+	const size_t b = partial_bytes(g);
+	return b ? static_cast<float2*>(scratch.partials.reserve(b)) : nullptr;
+}
+
 // Kernel arguments of a frame.
 DasArgs das_args(const FrameGeom& g, const void* d_tensor, const float* d_delay, const float* d_w, float* d_value,
-		float* d_factor)
+		float* d_factor, float2* d_partial)
 {
 	DasArgs a;
+	a.partial = d_partial;
 	a.tensor = d_tensor;
 	a.delay = d_delay;
 	a.w = d_w;
@@ -429,7 +447,8 @@ DasArgs das_args(const FrameGeom& g, const void* d_tensor, const float* d_delay,
 // ---------------------------------------------------------------- no allocation
 
 // Device scratch of the no-allocation DAS: the per-tile delay bounds, the
-// window words and the longest-window counter of one launch.  Every call on
+// window words and the longest-window counter of one launch, and, for more
+// than 128 transmits, the per-rx partial sums.  Every call on
 // a GPU shares that GPU's copy (each GPU the library runs on holds one until
 // the process ends) and rebuilds what it needs: calls that overlap on one GPU
 // race.  In host code `scratch` names a host-side stand-in; kernels get the
@@ -459,10 +478,11 @@ static unsigned char* noalloc_scratch()
 void launch_das_noalloc(const FrameGeom& g, int tuned_p0, int tuned_p1, DasScratch& last, const void* d_tensor,
 		const float* d_delay, const float* d_w, float* d_value, float* d_factor, hipStream_t s)
 {
-	DasArgs a = das_args(g, d_tensor, d_delay, d_w, d_value, d_factor);
 	if (!((g.ntx_pad <= 128 || g.ntx_pad % 8 == 0) && static_cast<uint64_t>(g.L) * storage_bytes(g.storage) < (1ull << 31)))
 		MISTA_THROW(MISTA_E_INVALID_PARAM, "This geometry has no tiled DAS form, which the no-allocation entry point needs.");
 	unsigned char* base = noalloc_scratch();
+	const size_t pb = (partial_bytes(g) + 255) / 256 * 256;   // per-rx sums between transmit blocks
+	DasArgs a = das_args(g, d_tensor, d_delay, d_w, d_value, d_factor, pb ? reinterpret_cast<float2*>(base) : nullptr);
 	// the tuned tile shape when it fits, else the default, else the larger tiles
 	std::vector<std::pair<int, int>> shapes;
 	if (tuned_p0 >= 0) shapes.emplace_back(tuned_p0, tuned_p1);
@@ -471,13 +491,13 @@ void launch_das_noalloc(const FrameGeom& g, int tuned_p0, int tuned_p1, DasScrat
 	for (const auto& sh : shapes) {
 		const TilePlan tp = tile_plan(g, sh.first);
 		if (!tp.ok) continue;
-		const size_t bb = (bounds_bytes(g, tp) + 255) / 256 * 256, wb = words_bytes(g, tp);
-		const size_t need = bb + wb + 256;
+		const size_t bb = (bounds_bytes(g, tp) + 255) / 256 * 256, wb = (words_bytes(g, tp) + 255) / 256 * 256;
+		const size_t need = pb + bb + wb + 256;
 		need_min = std::min(need_min, need);
 		if (need > kNoAllocScratchBytes) continue;
-		float2* bounds = reinterpret_cast<float2*>(base);
-		uint32_t* words = reinterpret_cast<uint32_t*>(base + bb);
-		uint32_t* d_max = reinterpret_cast<uint32_t*>(base + bb + wb);
+		float2* bounds = reinterpret_cast<float2*>(base + pb);
+		uint32_t* words = reinterpret_cast<uint32_t*>(base + pb + bb);
+		uint32_t* d_max = reinterpret_cast<uint32_t*>(base + pb + bb + wb);
 		const uint32_t S = build_window_tables(g, tp, d_delay, bounds, words, d_max, s);
 		if (launch_tiles(g, tp, sh.first, sh.second, S, words, a, last, s)) return;
 	}
@@ -553,7 +573,8 @@ uint32_t launch_das_batch(const std::vector<DasBatchItem>& items, hipStream_t s)
 		for (uint32_t k = 0; k < B.njobs; ++k) {
 			const Ready& r = ready[grp[k]];
 			const DasBatchItem& it = items[r.i];
-			const DasArgs a = das_args(*it.g, it.tensor, it.delay, it.w, it.value, it.factor);
+			const DasArgs a = das_args(*it.g, it.tensor, it.delay, it.w, it.value, it.factor,
+					scratch_partial(*it.g, *it.scratch));
 			B.jobs[k] = tile_args(*it.g, r.tp, a, it.scratch->params.as<const uint32_t>(), S, nbuf);
 			B.first[k] = total;
 			total += r.tp.ntiles;
@@ -580,7 +601,7 @@ size_t tensor_tail_padding() { return static_cast<size_t>(kMaxSlots) * 1024; }
 void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, const void* d_tensor,
 		const float* d_delay, const float* d_w, float* d_value, float* d_factor, hipStream_t s)
 {
-	const DasArgs a = das_args(g, d_tensor, d_delay, d_w, d_value, d_factor);
+	const DasArgs a = das_args(g, d_tensor, d_delay, d_w, d_value, d_factor, scratch_partial(g, scratch));
 
 	// window words hold ws * ESZ in 31 bits (L <= 2^20 samples)
 	const bool can_tile = (t.variant != 1) && (g.ntx_pad <= 128 || g.ntx_pad % 8 == 0) &&

@@ -111,6 +111,7 @@ struct DasArgs {
 	const float* w;       // [nrx][ntx]
 	float* value;         // [npix]
 	float* factor;        // [npix] or null
+	float2* partial;      // [nrx][npix] per-rx sums between transmit blocks (tiled, more than 128 transmits)
 	uint64_t npix;
 	uint32_t first_tx, ntx, nrx, L;
 	uint32_t w_stride;    // row stride of w (transmits padded)
@@ -472,10 +473,16 @@ __device__ __forceinline__ void das_tile(const TileArgs& A, const uint32_t tile,
 	const uint64_t npix = a.npix;
 	const uint64_t p = static_cast<uint64_t>(col) * A.tg.n2 + row;
 	const uint32_t L = a.L, ntx = a.ntx, nrx = a.nrx;
-	// transmits run in blocks of NTXMAX (MULTI: several, at NTXMAX = 128)
-	// (MULTI: several, the last one holding the rest in whole chunks)
+	// Transmits run in blocks: one of NTXMAX, or (MULTI, more than 128
+	// transmits) nblk equal blocks of BT <= 128, block-major over the receive
+	// elements, each receive element's partial sum kept in HBM from one block
+	// to the next (the same fma chain as one pass over all transmits), so a
+	// lane loads each block's transmit delays once.  A segment is one
+	// (block, rx): CPS chunks, the first of which also brings the tile's rx
+	// delays.
 	const uint32_t nblk = MULTI ? (A.ntx_pad + NTXMAX - 1) / NTXMAX : 1u;
-	const uint32_t CPR = MULTI ? A.ntx_pad / TC : CPB;   // chunks per rx
+	const uint32_t BT = MULTI ? A.ntx_pad / nblk : NTXMAX;   // transmits per block
+	const uint32_t CPS = MULTI ? BT / TC : CPB;               // chunks per segment
 	const float lim = static_cast<float>(L - 1u);
 	const uint32_t S = A.S, NB = A.nbuf;
 	const uint32_t wcap = S * SLOT;
@@ -488,45 +495,52 @@ __device__ __forceinline__ void das_tile(const TileArgs& A, const uint32_t tile,
 	const uint32_t lds0 = static_cast<uint32_t>(reinterpret_cast<size_t>((__attribute__((address_space(3))) void*)smem));
 
 	// tx delays of this lane's point for transmit block blk, offset folded
-	// in: (offset + d_tx).  Transmits are padded to whole blocks: padded ones
-	// have weight 0, position far below 0 and copy transmit ntx - 1's
-	// window, so they take the excluded path.
+	// in: (offset + d_tx).  Transmits are padded to whole chunks: padded ones
+	// have weight 0, position far below 0 and copy from the tensor's start,
+	// so they take the excluded path.
 	float dtx[NTXMAX];
 	auto load_dtx = [&](uint32_t blk) {
+		const int nvalid = static_cast<int>(min(BT, ntx - blk * BT));   // the block's transmits
 #pragma unroll
 		for (int t = 0; t < NTXMAX; ++t) {
-			const uint32_t tx = blk * NTXMAX + static_cast<uint32_t>(t);
-			const uint32_t tt = min(tx, ntx - 1);   // unconditional load, no branch
+			const uint32_t tt = min(blk * BT + static_cast<uint32_t>(t), ntx - 1);   // unconditional load, no branch
 			const float d = a.delay[static_cast<uint64_t>(a.first_tx + tt) * npix + p];
-			dtx[t] = (tx < ntx) ? a.offset + d : -1.0e30f;
+			// 0 for the block's transmits, <= -1e30 past them: arithmetic
+			// rather than a select, so no compare mask per transmit is held
+			// in scalar registers while the loads are in flight
+			const float pen = fminf(0.0f, static_cast<float>(nvalid - 1 - t)) * 1.0e30f;
+			dtx[t] = (a.offset + d) + pen;
 		}
 	};
 	load_dtx(0);
 
-	const uint32_t nchunks = nrx * CPR;
+	const uint32_t nchunks = nblk * nrx * CPS;
 	const uint32_t Q = PPW * S;   // window copies per wave per chunk (+1 for the rx delays at an rx start)
 	const uint32_t S4 = S & ~3u, SR = S & 3u;   // slots in groups of four, and the rest
-	// Window words and weights are laid out chunk after chunk: chunk k's
-	// pairs are entries k * TC .. k * TC + TC - 1 of the tile's table (the
-	// transmit rows are padded to whole chunks), and of the weights.
-	// Issue cursor: the next chunk to copy, its position in its rx,
-	// its ring buffer and the address of its first window's channel.
-	uint32_t is_k = 0, is_c = 0, is_lds = lds0;
-	const unsigned char* is_rbase = tensor;   // tensor + (rx of is_k) * chan_bytes
+	// Window words and weights are laid out [rx][ntx_pad] (the transmit rows
+	// padded to whole chunks): the chunk of segment (blk, r), position c
+	// holds entries r * ntx_pad + blk * BT + c * TC .. + TC - 1 of the tile's
+	// table and of the weights (without blocks: chunk k at k * TC).
+	// Issue cursor: the next chunk to copy, its segment position, its table
+	// offset, its ring buffer and the address of its first window's channel.
+	uint32_t is_k = 0, is_c = 0, is_r = 0, is_b = 0, is_off = 0, is_lds = lds0;
+	const unsigned char* is_rbase = tensor;   // tensor + (block start) * tx_bytes + (rx) * chan_bytes
 	const unsigned char* is_cbase = tensor;   // + is_c * TC * tx_bytes
 	const float* is_drx = a.delay + p;        // rx delays of is_k's rx for this lane
 	uint32_t iw[PPW];                         // words of is_k's pairs that this wave copies
 	auto load_issue_words = [&]() {
-		const cu32_t pr = wtab + is_k * TC + wave;
+		const cu32_t pr = wtab + is_off + wave;
 #pragma unroll
 		for (uint32_t q = 0; q < PPW; ++q) iw[q] = pr[q * NW];
 	};
+	uint32_t pend = 0;   // (MULTI) segment starts among the chunks issued and not yet computed
 	auto issue = [&]() {
+		if (MULTI && is_c == 0) ++pend;
 #pragma unroll
 		for (uint32_t k = 0; k < PPW; ++k) {
 			const uint32_t j = wave + k * NW;
 			// padded transmits copy from the tensor's start (weight 0)
-			const unsigned char* src = (is_c * TC + j < ntx) ? is_cbase + j * tx_bytes + (iw[k] & ~kEdgeBit) : tensor;
+			const unsigned char* src = (is_b * BT + is_c * TC + j < ntx) ? is_cbase + j * tx_bytes + (iw[k] & ~kEdgeBit) : tensor;
 			src += lane * 16;
 			uint32_t dst = is_lds + j * S * 1024u;
 			for (uint32_t s0 = 0; s0 < S4; s0 += 4, src += 4096, dst += 4096) lds_dma16_slots(src, dst, 4);
@@ -538,11 +552,21 @@ __device__ __forceinline__ void das_tile(const TileArgs& A, const uint32_t tile,
 		is_lds += buf_bytes;
 		if (is_lds == lds0 + NB * buf_bytes) is_lds = lds0;
 		is_cbase += TC * tx_bytes;
-		if (++is_c == CPR) {
+		is_off += TC;
+		if (++is_c == CPS) {
 			is_c = 0;
-			is_rbase += chan_bytes;
+			if (MULTI && ++is_r == nrx) {   // next block
+				is_r = 0;
+				++is_b;
+				is_rbase = tensor + static_cast<uint64_t>(is_b) * BT * tx_bytes;
+				is_drx = a.delay + p;
+				is_off = is_b * BT;
+			} else {
+				is_rbase += chan_bytes;
+				is_drx += npix;
+				is_off += A.ntx_pad - BT;   // (0 without blocks)
+			}
 			is_cbase = is_rbase;
-			is_drx += npix;
 		}
 	};
 	const uint32_t pre = min(NB - 1, nchunks);
@@ -554,9 +578,9 @@ __device__ __forceinline__ void das_tile(const TileArgs& A, const uint32_t tile,
 	// window words and weights of the chunk to compute, one chunk ahead
 	uint32_t cw[TC];
 	float cwt[TC];
-	auto load_chunk = [&](uint32_t k) {
-		const cu32_t pr = wtab + k * TC;
-		const cf32_t wr = (cf32_t)(a.w + static_cast<uint64_t>(k) * TC);
+	auto load_chunk = [&](uint32_t off) {
+		const cu32_t pr = wtab + off;
+		const cf32_t wr = (cf32_t)(a.w + static_cast<uint64_t>(off));
 #pragma unroll
 		for (uint32_t j = 0; j < TC; ++j) {
 			cw[j] = pr[j];
@@ -568,13 +592,31 @@ __device__ __forceinline__ void das_tile(const TileArgs& A, const uint32_t tile,
 	TileStats<CF> st;
 	st.m = reinterpret_cast<double*>(smem + NB * buf_bytes + tid * 32);
 	uint32_t i = 0, bbase = 0;   // chunk being computed, its buffer's LDS offset
-	for (uint32_t r = 0; r < nrx; ++r) {
-		float drx = 0.0f, ar = 0.0f, ai = 0.0f;
-		for (uint32_t blk = 0; blk < nblk; ++blk) {
-			if (MULTI) load_dtx(blk);   // waited for at its first use
+	for (uint32_t blk = 0; blk < nblk; ++blk) {
+		if (MULTI && blk > 0) load_dtx(blk);   // waited for at its first use
+		for (uint32_t r = 0; r < nrx; ++r) {
+			float drx = 0.0f, ar = 0.0f, ai = 0.0f;
+			if (MULTI && blk > 0) {   // the sums of the earlier blocks
+				const float2 ps = a.partial[static_cast<uint64_t>(r) * npix + p];
+				ar = ps.x;
+				ai = ps.y;
+			}
+			// (MULTI) table offset of the next chunk to load and of the next
+			// segment's first chunk: the next rx's, or the next block's
+			uint32_t ld_off = r * A.ntx_pad + blk * BT;
+			const uint32_t seg_next = r + 1 < nrx ? ld_off + A.ntx_pad : (blk + 1) * BT;
+			uint32_t left = CPS;   // (MULTI) chunks of this segment still to compute
 #pragma clang loop unroll(full)
-			for (int c = 0; c < static_cast<int>(CPB); ++c, ++i) {
-				if (MULTI && blk * CPB + static_cast<uint32_t>(c) >= CPR) break;   // the last block's end
+			for (int c = 0; c < static_cast<int>(CPB); ++c) {
+				// A block shorter than NTXMAX skips its last chunks: a guard,
+				// not an exit, so the trip count stays constant, the loop
+				// unrolls and dtx keeps compile-time indices.  The count is
+				// opaque to the compiler, which would otherwise hoist one
+				// compare per chunk out of the rx loop into scalar registers.
+				if (MULTI) {
+					asm volatile("" : "+s"(left));
+					if (left == 0) continue;
+				}
 				// this chunk's words and weights (loaded one chunk ahead)
 				uint32_t w_[TC];
 				float wt[TC];
@@ -583,16 +625,19 @@ __device__ __forceinline__ void das_tile(const TileArgs& A, const uint32_t tile,
 					w_[j] = cw[j];
 					wt[j] = cwt[j];
 				}
-				// copies issued after chunk i: chunks i+1 .. min(i+NB-2, last),
-				// Q each plus one for every chunk that starts an rx
-				const uint32_t cin = blk * CPB + static_cast<uint32_t>(c);   // chunk in the rx
+				// copies issued after chunk i: chunks i+1 .. is_k-1, Q each
+				// plus one for every chunk that starts a segment (loads and
+				// stores of partial sums issued since only make the wait
+				// stricter)
 				uint32_t after;
-				if (i + NB - 2 < nchunks) after = (NB - 2) * Q + (cin + NB - 2) / CPR;
+				if (MULTI) after = (is_k - 1 - i) * Q + pend - (c == 0 ? 1u : 0u);
+				else if (i + NB - 2 < nchunks) after = (NB - 2) * Q + (static_cast<uint32_t>(c) + NB - 2) / CPS;
 				else after = (nchunks - 1 - i) * Q + (nrx - 1 - r);
 				wait_vmcnt_at_most(after);         // this wave's copies of chunk i landed
 				__syncthreads();                   // everyone's did; buffer of chunk i-1 is free
+				if (MULTI && c == 0) --pend;
 				if (is_k < nchunks) issue();
-				if (c == 0 && blk == 0) drx = *reinterpret_cast<const float*>(smem + bbase + win_bytes + tid * 4);
+				if (c == 0) drx = *reinterpret_cast<const float*>(smem + bbase + win_bytes + tid * 4);
 				// Register fences: this chunk's positions are computed after
 				// the barrier, and its accumulation is finished before the
 				// next one, so only one chunk's temporaries are ever live.
@@ -670,12 +715,25 @@ __device__ __forceinline__ void das_tile(const TileArgs& A, const uint32_t tile,
 				// scalar load in flight would hold up their wait): in flight
 				// during the next barrier
 				if (is_k < nchunks) load_issue_words();
-				if (i + 1 < nchunks) load_chunk(i + 1);
+				if (i + 1 < nchunks) {
+					if (MULTI) {
+						ld_off = left > 1 ? ld_off + TC : seg_next;
+						load_chunk(ld_off);
+					} else {
+						load_chunk((i + 1) * TC);
+					}
+				}
 				bbase += buf_bytes;
 				if (bbase == NB * buf_bytes) bbase = 0;
+				++i;
+				if (MULTI) --left;
+			}
+			if (MULTI && blk + 1 < nblk) {
+				if (valid) a.partial[static_cast<uint64_t>(r) * npix + p] = make_float2(ar, ai);
+			} else {
+				st.add_rx(ar, ai, r == 0);
 			}
 		}
-		st.add_rx(ar, ai, r == 0);
 	}
 	MISTA_VMCNT(0);
 	if (valid) st.write(a, p);
@@ -757,10 +815,9 @@ void launch_tile_t(const TileBatch& B, uint32_t nt, uint32_t tc, hipStream_t s)
 #define MISTA_GO(tcv, ntv, multi) \
 	go1(das_tile_kernel<STORAGE, CF, NTXMAX, tcv, ntv, multi>, das_tile_batch_kernel<STORAGE, CF, NTXMAX, tcv, ntv, multi>, ntv)
 	if constexpr (NTXMAX == 128) {
-		if (A.ntx_pad > NTXMAX) {   // several transmit blocks
-			if (nt == 512) MISTA_GO(8, 512, true);
-			else if (tc == 8) MISTA_GO(8, 256, true);
-			else MISTA_GO(4, 256, true);
+		if (A.ntx_pad > NTXMAX) {   // several transmit blocks: 512-lane tiles only
+			if (nt != 512) MISTA_THROW(MISTA_E_INVALID_STATE, "tile of " << nt << " lanes for " << A.ntx_pad << " transmits");
+			MISTA_GO(8, 512, true);
 			return;
 		}
 	}

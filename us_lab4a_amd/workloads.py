@@ -10,6 +10,8 @@ sta2d_128_pcf    128 elements, pitch 0.3 mm, 5 MHz pulse, fs 40 MHz,
                  speckle (5000 scatterers/cm^2 over the image, Rayleigh
                  amplitudes), AWGN 30 dB SNR; U = 8, PCF (gamma = 1), Hann rx
                  apodization; grid 512 x 1024 over x [-19, 19] mm, z [5, 65] mm.
+sta2d_192_long   192 elements, 8192 samples, U = 8 (FFT size 67 500), PCF,
+                 512 x 512 grid: the serving mix's long records.
 sta3d_sparse     32 x 32 matrix array (1024 rx), pitch 0.3 mm, 64 seeded tx
                  elements, fs 40 MHz, 3 MHz pulse, 64 x 1024 x 4096 samples from
                  500 scatterers in a 20 x 20 x 60 mm volume, 35 dB SNR; U = 4,
@@ -243,7 +245,19 @@ def sta3d_small(cf="sign_coherence_factor", n=(12, 10, 24)):
                       points=6)
 
 
-WORKLOADS = {"sta2d_32_small": sta2d_32_small, "sta2d_128_pcf": sta2d_128_pcf, "sta3d_sparse": sta3d_sparse}
+def sta2d_serve_long(num_elements=192):
+    """The serving mix's long records: 192 elements, 8192 samples, U = 8
+    (FFT size 67 500: the four-step preparation), PCF, grid 512 x 512 over the
+    aperture's width and z 5..65 mm, points plus speckle (2000 / cm^2)."""
+    half = num_elements * 0.3 * MM / 2
+    return Workload2D(name=f"sta2d_{num_elements}_long", num_elements=num_elements, num_samples=8192, upsampling=8,
+                      cf="phase_coherence_factor", grid_x=(-half, half), grid_z=(5 * MM, 65 * MM), n_x=512, n_z=512,
+                      points=50, point_x=(-half, half), point_z=(5 * MM, 65 * MM), speckle_density=2000.0,
+                      seed_points=492, seed_speckle=592, seed_noise=692)
+
+
+WORKLOADS = {"sta2d_32_small": sta2d_32_small, "sta2d_128_pcf": sta2d_128_pcf, "sta3d_sparse": sta3d_sparse,
+             "sta2d_192_long": sta2d_serve_long}
 
 
 def db_error(env, env_ref):
