@@ -372,6 +372,55 @@ template<> __device__ __forceinline__ void dft<5>(float2* v)
 	v[3] = csub(t2, mul_mi(u2));
 }
 
+// Composite radices: 6 = 2 x 3 and 9 = 3 x 3 in registers (Cooley-Tukey,
+// inner twiddles exp(-2 pi i n k / R) as constants), so that one LDS pass
+// does the work of two.
+template<> __device__ __forceinline__ void dft<6>(float2* v)
+{
+	constexpr float h = 0.86602540378443864676f;   // sin(2 pi / 6)
+	float2 e[3] = {v[0], v[2], v[4]}, o[3] = {v[1], v[3], v[5]};
+	dft<3>(e);
+	dft<3>(o);
+	// o[k] * exp(-2 pi i k / 6): k = 1: (1/2, -h); k = 2: (-1/2, -h)
+	const float2 o1 = make_float2(0.5f * o[1].x + h * o[1].y, 0.5f * o[1].y - h * o[1].x);
+	const float2 o2 = make_float2(-0.5f * o[2].x + h * o[2].y, -0.5f * o[2].y - h * o[2].x);
+	v[0] = cadd(e[0], o[0]);
+	v[3] = csub(e[0], o[0]);
+	v[1] = cadd(e[1], o1);
+	v[4] = csub(e[1], o1);
+	v[2] = cadd(e[2], o2);
+	v[5] = csub(e[2], o2);
+}
+template<> __device__ __forceinline__ void dft<9>(float2* v)
+{
+	// columns n1 = 0..2: DFT_3 over n2 of v[n1 + 3 n2]
+	float2 a[3][3];
+#pragma unroll
+	for (int n1 = 0; n1 < 3; ++n1) {
+		a[n1][0] = v[n1];
+		a[n1][1] = v[n1 + 3];
+		a[n1][2] = v[n1 + 6];
+		dft<3>(a[n1]);
+	}
+	// twiddles exp(-2 pi i n1 k2 / 9): (n1, k2) = (1,1), (1,2), (2,1), (2,2)
+	constexpr float c1 = 0.76604444311897803520f, s1 = 0.64278760968653932632f;   // 40 degrees
+	constexpr float c2 = 0.17364817766693034885f, s2 = 0.98480775301220805936f;   // 80 degrees
+	constexpr float c4 = -0.93969262078590838405f, s4 = 0.34202014332566873304f;  // 160 degrees
+	a[1][1] = cmul(a[1][1], make_float2(c1, -s1));
+	a[1][2] = cmul(a[1][2], make_float2(c2, -s2));
+	a[2][1] = cmul(a[2][1], make_float2(c2, -s2));
+	a[2][2] = cmul(a[2][2], make_float2(c4, -s4));
+	// rows k2: DFT_3 over n1, output k = k2 + 3 k1
+#pragma unroll
+	for (int k2 = 0; k2 < 3; ++k2) {
+		float2 b[3] = {a[0][k2], a[1][k2], a[2][k2]};
+		dft<3>(b);
+		v[k2] = b[0];
+		v[k2 + 3] = b[1];
+		v[k2 + 6] = b[2];
+	}
+}
+
 template<int R>
 __device__ __forceinline__ void fft_pass(float2* D, uint32_t M, uint32_t Ns, uint32_t magic,
 		const float2* __restrict__ tw)
@@ -397,9 +446,11 @@ __device__ __forceinline__ void fft_pass(float2* D, uint32_t M, uint32_t Ns, uin
 			// and b e < 2^32 (b, e < 2^15) keeps the floor exact
 			const uint32_t q = (Ns == 1) ? b : __umulhi(b, magic);
 			const uint32_t j = b - q * Ns;
-			const float2* w = tw + j * (R - 1);
+			if (Ns > 1) {   // (the first pass's twiddles are all 1)
+				const float2* w = tw + j * (R - 1);
 #pragma unroll
-			for (int i = 1; i < R; ++i) v[t][i] = cmul(v[t][i], w[i - 1]);
+				for (int i = 1; i < R; ++i) v[t][i] = cmul(v[t][i], w[i - 1]);
+			}
 			dft<R>(v[t]);
 			const uint32_t o = q * Ns * R + j;
 #pragma unroll
@@ -415,6 +466,8 @@ __device__ void fft_forward(float2* D, const FftPlanArgs& P, const float2* __res
 		const float2* t = tw + P.tw_off[p];
 		switch (P.radix[p]) {
 		case 8: fft_pass<8>(D, P.M, P.ns[p], P.ns_magic[p], t); break;
+		case 9: fft_pass<9>(D, P.M, P.ns[p], P.ns_magic[p], t); break;
+		case 6: fft_pass<6>(D, P.M, P.ns[p], P.ns_magic[p], t); break;
 		case 4: fft_pass<4>(D, P.M, P.ns[p], P.ns_magic[p], t); break;
 		case 2: fft_pass<2>(D, P.M, P.ns[p], P.ns_magic[p], t); break;
 		case 3: fft_pass<3>(D, P.M, P.ns[p], P.ns_magic[p], t); break;
@@ -989,8 +1042,8 @@ static size_t fused_lds_bytes(const FrameGeom& g, uint32_t J, uint32_t off)
 	return b <= 160 * 1024 - 512 ? b : 0;
 }
 
-// Radix plan (5s, one 8, 3s, 8s, 4, 2 in that order) and twiddles of the size-M
-// transforms, computed in double and rounded once.
+// Radix plan (5s, one 8, 9s, a 6 or 3, 8s, 4, 2 in that order) and twiddles of
+// the size-M transforms, computed in double and rounded once.
 static void build_fused_plan(PrepPlan& P, uint32_t nf, hipStream_t s)
 {
 	const uint32_t M = nf / 2;
@@ -1003,10 +1056,15 @@ static void build_fused_plan(PrepPlan& P, uint32_t nf, hipStream_t s)
 	// over the banks (an even one piles them onto a few); one 8 next makes
 	// Ns a multiple of 8, whose runs of contiguous outputs collide least in
 	// the following passes (counted for M = 17280: 8648 conflict cycles per
-	// transform against 16200 for 8, 8, 2, 3, 3, 3, 5).
+	// transform against 16200 for 8, 8, 2, 3, 3, 3, 5).  Pairs of 3s go as
+	// one radix-9 pass and a lone 3 with a 2 as radix 6 (M = 17280: 5, 8, 9,
+	// 6, 8, five passes instead of seven), each pass costing two barriers
+	// and an LDS round trip.
 	std::vector<uint32_t> rad;
 	while (m % 5 == 0) { rad.push_back(5); m /= 5; }
 	if (m % 8 == 0) { rad.push_back(8); m /= 8; }
+	while (m % 9 == 0) { rad.push_back(9); m /= 9; }
+	if (m % 6 == 0) { rad.push_back(6); m /= 6; }
 	while (m % 3 == 0) { rad.push_back(3); m /= 3; }
 	while (m % 8 == 0) { rad.push_back(8); m /= 8; }
 	while (m % 4 == 0) { rad.push_back(4); m /= 4; }
@@ -1049,7 +1107,9 @@ static void prepare_fused(PrepState& st, const FrameGeom& g, const float* d_raw,
 		uint32_t J, uint32_t off, void* d_tensor, float* d_scale, size_t lds, hipStream_t s)
 {
 	const uint32_t nch = g.ntx * g.nrx;
-	const uint32_t chunk = std::min<uint32_t>(nch, 1024u);
+	// 1024-channel chunks when the host uploads each chunk just in time, else
+	// every channel in one launch
+	const uint32_t chunk = st.before_chunk ? std::min<uint32_t>(nch, 1024u) : nch;
 	if (!st.plan) st.plan = new PrepPlan();
 	PrepPlan& P = *st.plan;
 	if (P.fused_nf != g.nf) build_fused_plan(P, g.nf, s);
