@@ -1148,6 +1148,92 @@ static void prepare_fused(PrepState& st, const FrameGeom& g, const float* d_raw,
 	}
 }
 
+// FIR x UP + DC removal with the fused kernel's structure (the 27-tap
+// design at UP = 4 or 8): a lane per input position q holds the window
+// x[q-26..q] in registers and produces the UP outputs m = q UP + p
+// (i = m - off), the taps the same for every lane through the scalar cache;
+// the lane's QPT x UP outputs wait in registers for the mean.  Each output is
+// the same fma chain over j = 0..J-1 as fir_dc_kernel's, which reads both
+// operands from LDS per tap.
+template<int UP, int QPT>
+__global__ void __launch_bounds__(kFirThreads) fir_dc_q_kernel(const float* __restrict__ raw, uint32_t ns,
+		uint32_t L, uint32_t nf, uint32_t dz, const float* __restrict__ hpoly, uint32_t off, float* __restrict__ out)
+{
+	extern __shared__ float smem[];
+	constexpr uint32_t J = kFirJ;
+	const uint32_t ch = blockIdx.x;
+	const uint32_t tid = threadIdx.x;
+	const uint32_t right = off / UP + 2;
+	float* xs = smem;   // [J + ns + right], J zeros on the left
+	__shared__ double red[kFirThreads / 64];
+
+	const float* src = raw + static_cast<uint64_t>(ch) * ns;
+	for (uint32_t i = tid; i < J + ns + right; i += kFirThreads) {
+		const int q = static_cast<int>(i) - static_cast<int>(J);
+		xs[i] = (q >= 0 && q < static_cast<int>(ns)) ? src[q] : 0.0f;
+	}
+	__syncthreads();
+
+	const cf32_t hg = (cf32_t)hpoly;
+	const uint32_t q_lo = off / UP, q_hi = (L - 1 + off) / UP;
+	float y[QPT][UP];
+	double part = 0.0;
+#pragma unroll
+	for (int k = 0; k < QPT; ++k) {
+		const uint32_t q = q_lo + tid + static_cast<uint32_t>(k) * kFirThreads;
+		if (q <= q_hi) {
+			float xw[J];
+			const float* xrow = xs + J + q;
+#pragma unroll
+			for (uint32_t j = 0; j < J; ++j) xw[j] = xrow[-static_cast<int>(j)];
+			const int ibase = static_cast<int>(q * UP) - static_cast<int>(off);
+#pragma unroll
+			for (int ph = 0; ph < UP; ++ph) {
+				const cf32_t hrow = hg + ph * J;
+				float acc = 0.0f;
+#pragma unroll
+				for (uint32_t j = 0; j < J; ++j) acc = fmaf(hrow[j], xw[j], acc);
+				const int i = ibase + ph;
+				if (i >= static_cast<int>(dz) && i < static_cast<int>(L)) part += static_cast<double>(acc);
+				y[k][ph] = acc;
+			}
+		}
+	}
+	// block reduction of the DC sum (fixed order: lanes, then waves)
+	for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
+	if ((tid & 63) == 0) red[tid >> 6] = part;
+	__syncthreads();
+	if (tid == 0) {
+		double sum = 0.0;
+		for (int w = 0; w < kFirThreads / 64; ++w) sum += red[w];
+		red[0] = sum;
+	}
+	__syncthreads();
+	const float mean = static_cast<float>(red[0] / static_cast<double>(L - dz));
+
+	float* dst = out + static_cast<uint64_t>(ch) * nf;
+#pragma unroll
+	for (int k = 0; k < QPT; ++k) {
+		const uint32_t q = q_lo + tid + static_cast<uint32_t>(k) * kFirThreads;
+		if (q > q_hi) continue;
+		const int ibase = static_cast<int>(q * UP) - static_cast<int>(off);
+		// a lane's UP outputs are contiguous: 16-byte stores where aligned
+		if (ibase >= 0 && ibase + UP <= static_cast<int>(L) && (ibase & 3) == 0) {
+#pragma unroll
+			for (int h = 0; h < UP; h += 4)
+				*reinterpret_cast<float4*>(dst + ibase + h) =
+						make_float4(y[k][h] - mean, y[k][h + 1] - mean, y[k][h + 2] - mean, y[k][h + 3] - mean);
+		} else {
+#pragma unroll
+			for (int ph = 0; ph < UP; ++ph) {
+				const int i = ibase + ph;
+				if (i >= 0 && i < static_cast<int>(L)) dst[i] = y[k][ph] - mean;
+			}
+		}
+	}
+	for (uint32_t i = L + tid; i < nf; i += kFirThreads) dst[i] = 0.0f;
+}
+
 // FIR x U + DC removal of n channels into rows of Nf floats (zero padded).
 static void launch_fir_any(uint32_t n, const FrameGeom& g, const float* raw, const float* hpoly, uint32_t J,
 		uint32_t off, float* real, hipStream_t s)
@@ -1157,6 +1243,29 @@ static void launch_fir_any(uint32_t n, const FrameGeom& g, const float* raw, con
 		if (lds > 150 * 1024) MISTA_THROW(MISTA_E_INVALID_PARAM, "channel too long for the LDS FIR: " << g.ns << " samples");
 	}
 	const uint32_t vpt = (g.L + kFirThreads - 1) / kFirThreads;
+	// the lane-per-input kernel: the 27-tap design at U = 4 or 8, every
+	// output of a lane in registers (at most 64)
+	const uint32_t nq = g.up > 1 ? (g.L - 1 + off) / g.up - off / g.up + 1 : 0;
+	const uint32_t qpt = (nq + kFirThreads - 1) / kFirThreads;
+	if (J == kFirJ && (g.up == 8 || g.up == 4) && qpt * g.up <= 64) {
+		const size_t lds = (static_cast<size_t>(J) + g.ns + off / g.up + 2) * sizeof(float);
+		auto go = [&](auto kern) {
+			HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+					static_cast<int>(lds)));
+			hipLaunchKernelGGL(kern, dim3(n), dim3(kFirThreads), lds, s, raw, g.ns, g.L, g.nf, g.dz_up, hpoly, off, real);
+		};
+		if (g.up == 8) {
+			if (qpt <= 2) go(fir_dc_q_kernel<8, 2>);
+			else if (qpt <= 4) go(fir_dc_q_kernel<8, 4>);
+			else go(fir_dc_q_kernel<8, 8>);
+		} else {
+			if (qpt <= 4) go(fir_dc_q_kernel<4, 4>);
+			else if (qpt <= 8) go(fir_dc_q_kernel<4, 8>);
+			else go(fir_dc_q_kernel<4, 16>);
+		}
+		HIP_CHECK(hipGetLastError());
+		return;
+	}
 	if (g.up == 1) {
 		hipLaunchKernelGGL(copy_dc_kernel, dim3(n), dim3(kFirThreads), 0, s, raw, g.L, g.nf, g.dz_up, real);
 	} else if (vpt <= 4) {
