@@ -619,28 +619,41 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 					(static_cast<uint64_t>(g.nrx) << 50) ^ (static_cast<uint64_t>(g.storage) << 60) ^
 					(static_cast<uint64_t>(g.cf_kind) << 62) ^ static_cast<uint64_t>(g.offset * 1024.0f);
 			if (scratch.tuned_key != key) {
-				const std::vector<int> shapes = g.ny > 1 ? std::vector<int>{4, 7} : std::vector<int>{1, 2, 3, 5, 6};
+				const std::vector<int> shapes = g.ny > 1 ? std::vector<int>{7, 4} : std::vector<int>{1, 2, 3, 5, 6};
 				hipEvent_t e0 = nullptr, e1 = nullptr;
 				HIP_CHECK(hipEventCreate(&e0));
 				HIP_CHECK(hipEventCreate(&e1));
 				float best = 0.0f;
 				int bp0 = -1, bp1 = 0;
-				// frames this long gain little from tuning: the first fitting
-				// configuration is kept once a run takes longer than this
+				// frames this long are not searched chunking by chunking: once a
+				// run takes longer than this, each remaining tile shape gets
+				// one timed run of its first fitting configuration
 				constexpr float kTuneLimitMs = 150.0f;
+				bool long_frame = false;
 				int last_p0 = -1;   // shape whose tables are resident
+				auto timed = [&](int p0, int p1, float& ms) {
+					HIP_CHECK(hipEventRecord(e0, s));
+					const bool run = launch_tiled(g, p0, p1, scratch, a, d_delay, s);
+					HIP_CHECK(hipEventRecord(e1, s));
+					HIP_CHECK(hipEventSynchronize(e1));
+					HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+					return run;
+				};
 				for (int p0 : shapes) {
-					if (bp0 >= 0 && best > kTuneLimitMs) break;
 					// 256-lane tiles: three or two workgroups per CU; 512-lane
 					// tiles: one
 					const int kbs[2] = {p0 >= 5 ? 76 : 52, p0 >= 5 ? 150 : 76};
-					// untimed for comparison (it builds the shape's tables), but
-					// a first run this long ends the search with its own result
+					// untimed for comparison (it builds the shape's tables)
 					HIP_CHECK(hipEventRecord(e0, s));
 					// the last try takes the smallest LDS budget the windows
 					// fit, up to a CU's whole LDS (long windows: steep angles,
-					// coarse grids)
-					const int tries[4] = {8 | (kbs[0] << 16), 4 | (kbs[0] << 16), kbs[1] << 16, 0};
+					// coarse grids); 512-lane tiles, one per CU whatever the
+					// budget, try the whole LDS first
+					int tries[4] = {8 | (kbs[0] << 16), 4 | (kbs[0] << 16), kbs[1] << 16, 0};
+					if (p0 >= 5) {
+						tries[0] = 8 | (kbs[1] << 16);
+						tries[1] = 8 | (kbs[0] << 16);
+					}
 					int built = 0;
 					bool ok = false;
 					for (int k = 0; k < 4 && !ok; ++k) {
@@ -653,11 +666,15 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 					HIP_CHECK(hipEventSynchronize(e1));
 					float first_ms = 0.0f;
 					HIP_CHECK(hipEventElapsedTime(&first_ms, e0, e1));
-					if (bp0 < 0 && first_ms > 2.0f * kTuneLimitMs) {
-						best = first_ms;
-						bp0 = p0;
-						bp1 = built;
-						break;
+					if (first_ms > 2.0f * kTuneLimitMs) long_frame = true;
+					if (long_frame) {
+						float ms = 0.0f;
+						if (timed(p0, built, ms) && (bp0 < 0 || ms < best)) {
+							best = ms;
+							bp0 = p0;
+							bp1 = built;
+						}
+						continue;
 					}
 					// pairs per chunk (8 first: the better one when it fits) in
 					// each budget, then the smallest budget that fits
@@ -666,7 +683,10 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 						for (int tc : {8, 4}) cands.push_back(tc | (kb << 16));
 					cands.push_back(0);
 					for (const int p1 : cands) {
-						if (bp0 >= 0 && best > kTuneLimitMs) break;
+						if (bp0 >= 0 && best > kTuneLimitMs) {
+							long_frame = true;
+							break;
+						}
 						// the better of two runs (the first may find cold caches)
 						float ms = 0.0f;
 						bool run = false;
