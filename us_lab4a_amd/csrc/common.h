@@ -147,7 +147,11 @@ Fft4Plan* fft4_plan(uint32_t n, hipStream_t s);
 void fft4_destroy(Fft4Plan* p);
 uint32_t fft4_size(const Fft4Plan* p);
 size_t fft4_device_bytes(const Fft4Plan* p);
-void fft4_exec(const Fft4Plan& p, const float2* in, float2* tmp, float2* out, uint32_t batch, hipStream_t s);
+// ymax (optional): per transform, atomic max of the bits of the largest
+// |Re X[k]|, |Im X[k]| over k < npair (and |Re X[npair]| when odd); zeroed by
+// the caller beforehand.
+void fft4_exec(const Fft4Plan& p, const float2* in, float2* tmp, float2* out, uint32_t batch, hipStream_t s,
+		uint32_t* ymax = nullptr, uint32_t npair = 0, bool odd = false);
 struct PrepState {
 	PrepPlan* plan = nullptr;  // owned; destroyed in prep.hip where PrepPlan is complete
 	int method = 0;            // 0 automatic (fused kernel, else four-step FFTs), 1 library FFT (hipFFT), 2 fused kernel, 3 four-step FFTs

@@ -158,6 +158,11 @@ struct F4Args {
 	const float2* tw1;     // Stockham twiddles of the N1-point transforms
 	const float2* tw2;     // of the N2-point transforms
 	const float2* twn;     // exp(-2 pi i k / N), k < N
+	// optional (pass B): per transform, the largest |Re X[k]|, |Im X[k]| over
+	// k < npair, and |Re X[npair]| when odd, merged into ymax[transform] as
+	// the bits of a non-negative float (atomic max; zeroed beforehand)
+	uint32_t* ymax;
+	uint32_t npair, odd;
 };
 
 // Pass A: grid (column blocks, transforms).
@@ -202,9 +207,24 @@ __global__ void __launch_bounds__(kF4Threads) f4_rows_kernel(const float2* __res
 	__syncthreads();
 	const float2* res = f4_short(a, b, cnt, A.s2, A.tw2);
 	// X[k1 + N1 k2], k1 = r0 + r: consecutive lanes on consecutive k1
+	float mx = 0.0f;
 	for (uint32_t i = threadIdx.x; i < n2 * cnt; i += kF4Threads) {
 		const uint32_t k2 = i / cnt, r = i - k2 * cnt;
-		X[base + static_cast<uint64_t>(k2) * n1 + r0 + r] = res[r * n2 + k2];
+		const float2 v = res[r * n2 + k2];
+		const uint32_t k = k2 * n1 + r0 + r;
+		X[base + k] = v;
+		if (k < A.npair) mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
+		else if (A.odd && k == A.npair) mx = fmaxf(mx, fabsf(v.x));
+	}
+	if (A.ymax) {
+		__shared__ float red[kF4Threads / 64];
+		for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+		if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			for (int w = 1; w < kF4Threads / 64; ++w) mx = fmaxf(mx, red[w]);
+			atomicMax(A.ymax + blockIdx.y, __float_as_uint(mx));   // non-negative floats order as their bits
+		}
 	}
 }
 
@@ -295,9 +315,11 @@ size_t fft4_device_bytes(const Fft4Plan* p) { return p ? p->tw.capacity() : 0u; 
 
 // batch forward transforms of size n: in ([batch][n] complex) -> out
 // ([batch][n]) through tmp ([batch][n]); out may alias in.
-void fft4_exec(const Fft4Plan& P, const float2* in, float2* tmp, float2* out, uint32_t batch, hipStream_t s)
+void fft4_exec(const Fft4Plan& P, const float2* in, float2* tmp, float2* out, uint32_t batch, hipStream_t s,
+		uint32_t* ymax, uint32_t npair, bool odd)
 {
 	F4Args A{};
+	A.ymax = nullptr;
 	A.n1 = P.n1;
 	A.n2 = P.n2;
 	A.n = P.n;
@@ -314,6 +336,9 @@ void fft4_exec(const Fft4Plan& P, const float2* in, float2* tmp, float2* out, ui
 	hipLaunchKernelGGL(f4_cols_kernel, dim3((P.n2 + P.cb - 1) / P.cb, batch), dim3(kF4Threads), lds_a, s, in, tmp, A);
 	HIP_CHECK(hipGetLastError());
 	A.block = P.rb;
+	A.ymax = ymax;
+	A.npair = npair;
+	A.odd = odd ? 1u : 0u;
 	const size_t lds_b = 2 * static_cast<size_t>(P.rb) * P.n2 * sizeof(float2);
 	HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(f4_rows_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
 			static_cast<int>(lds_b)));

@@ -121,6 +121,20 @@ void launch_delay_table_3d(const float* d_grid_xyz, uint64_t npix, const double*
 
 constexpr int kFirThreads = 1024;
 
+// Largest |stored value| of a channel's FIR output (non-negative floats), a
+// 1024-lane workgroup's reduction; thread 0 writes it.  The four-step path's
+// FP16/BF16 scale takes it from here instead of a second pass over the row.
+__device__ __forceinline__ void store_channel_max(float m, float* __restrict__ dst, float* red)
+{
+	for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+	if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int w = 1; w < kFirThreads / 64; ++w) m = fmaxf(m, red[w]);
+		*dst = m;
+	}
+}
+
 // One workgroup per channel.  The raw channel sits in LDS with J zeros on the
 // left and enough on the right, the polyphase filter hp[phase][j] =
 // h[phase + U*j] (zero padded to J taps per phase) follows it.  Output
@@ -129,9 +143,10 @@ constexpr int kFirThreads = 1024;
 template<int VPT>
 __global__ void __launch_bounds__(kFirThreads) fir_dc_kernel(const float* __restrict__ raw, uint32_t ns,
 		uint32_t up, uint32_t L, uint32_t nf, uint32_t dz, const float* __restrict__ hpoly, uint32_t J,
-		uint32_t off, float* __restrict__ out)
+		uint32_t off, float* __restrict__ out, float* __restrict__ xmax)
 {
 	extern __shared__ float smem[];
+	__shared__ float redm[kFirThreads / 64];
 	const uint32_t ch = blockIdx.x;
 	const uint32_t tid = threadIdx.x;
 	const uint32_t right = off / up + 2;
@@ -177,19 +192,25 @@ __global__ void __launch_bounds__(kFirThreads) fir_dc_kernel(const float* __rest
 	const float mean = static_cast<float>(red[0] / static_cast<double>(L - dz));
 
 	float* dst = out + static_cast<uint64_t>(ch) * nf;
+	float mx = 0.0f;
 #pragma unroll
 	for (int v = 0; v < VPT; ++v) {
 		const uint32_t i = tid + static_cast<uint32_t>(v) * kFirThreads;
-		if (i < L) dst[i] = y[v] - mean;
+		if (i < L) {
+			dst[i] = y[v] - mean;
+			mx = fmaxf(mx, fabsf(y[v] - mean));
+		}
 	}
 	for (uint32_t i = L + tid; i < nf; i += kFirThreads) dst[i] = 0.0f;
+	if (xmax) store_channel_max(mx, xmax + ch, redm);
 }
 
 // U = 1: copy + DC removal (Util::removeDC only).
 __global__ void __launch_bounds__(kFirThreads) copy_dc_kernel(const float* __restrict__ raw, uint32_t L,
-		uint32_t nf, uint32_t dz, float* __restrict__ out)
+		uint32_t nf, uint32_t dz, float* __restrict__ out, float* __restrict__ xmax)
 {
 	__shared__ double red[kFirThreads / 64];
+	__shared__ float redm[kFirThreads / 64];
 	const uint32_t ch = blockIdx.x;
 	const uint32_t tid = threadIdx.x;
 	const float* src = raw + static_cast<uint64_t>(ch) * L;
@@ -206,7 +227,13 @@ __global__ void __launch_bounds__(kFirThreads) copy_dc_kernel(const float* __res
 	__syncthreads();
 	const float mean = static_cast<float>(red[0] / static_cast<double>(L - dz));
 	float* dst = out + static_cast<uint64_t>(ch) * nf;
-	for (uint32_t i = tid; i < nf; i += kFirThreads) dst[i] = (i < L) ? src[i] - mean : 0.0f;
+	float mx = 0.0f;
+	for (uint32_t i = tid; i < nf; i += kFirThreads) {
+		const float v = (i < L) ? src[i] - mean : 0.0f;
+		dst[i] = v;
+		mx = fmaxf(mx, fabsf(v));
+	}
+	if (xmax) store_channel_max(mx, xmax + ch, redm);
 }
 
 // ---------------------------------------------------------------- Hilbert
@@ -741,9 +768,10 @@ __global__ void __launch_bounds__(kFftThreads) analytic_lds_kernel(const Analyti
 // conj(Z[M-k]), C[M/2] = i Z[M/2]) over rows of M complex values; lane k
 // owns the pair (k, M-k), k <= M/2.  wpost[k] = exp(-2 pi i k / N).
 __global__ void __launch_bounds__(256) hilbert_pairs_kernel(float2* __restrict__ z, uint32_t M,
-		const float2* __restrict__ wpost)
+		const float2* __restrict__ wpost, uint32_t* __restrict__ ymax)
 {
 	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k == 0 && ymax) ymax[blockIdx.y] = 0u;   // the inverse transform's maximum starts here
 	if (k > M / 2) return;
 	float2* D = z + static_cast<uint64_t>(blockIdx.y) * M;
 	if (k == 0) {
@@ -769,9 +797,9 @@ __global__ void __launch_bounds__(256) hilbert_pairs_kernel(float2* __restrict__
 // if zero).  One workgroup per channel; lane per sample pair.
 template<int STORAGE>
 __global__ void __launch_bounds__(256) store_xy_kernel(const float* __restrict__ x, const float2* __restrict__ F,
-		uint32_t N, uint32_t L, void* __restrict__ tensor, float* __restrict__ scale, uint64_t ch0)
+		uint32_t N, uint32_t L, void* __restrict__ tensor, float* __restrict__ scale, uint64_t ch0,
+		const float* __restrict__ xmax, const uint32_t* __restrict__ ymax)
 {
-	__shared__ float red[4];
 	const uint32_t ch = blockIdx.x, tid = threadIdx.x, M = N / 2;
 	const float2* xr = reinterpret_cast<const float2*>(x + static_cast<uint64_t>(ch) * N);
 	const float2* fr = F + static_cast<uint64_t>(ch) * M;
@@ -781,16 +809,10 @@ __global__ void __launch_bounds__(256) store_xy_kernel(const float* __restrict__
 	const bool odd = (L & 1u) != 0;   // the last sample has no partner
 	float inv = 1.0f;
 	if constexpr (STORAGE != MISTA_STORAGE_F32) {
-		float mx = 0.0f;
-		for (uint32_t m = tid; m < npair; m += 256) {
-			const float2 xv = xr[m], f = fr[m];
-			mx = fmaxf(fmaxf(mx, fmaxf(fabsf(xv.x), fabsf(xv.y))), fmaxf(fabsf(f.x * coef), fabsf(-f.y * coef)));
-		}
-		if (odd && tid == 0) mx = fmaxf(mx, fmaxf(fabsf(xr[npair].x), fabsf(fr[npair].x * coef)));
-		for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-		if ((tid & 63) == 0) red[tid >> 6] = mx;
-		__syncthreads();
-		const float gm = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+		// max |x| of the FIR row and max |Re F|, |Im F| of the transform, as
+		// their kernels left them: scaling by coef > 0 rounds monotonically,
+		// so max |F| * coef is the largest |y|
+		const float gm = fmaxf(xmax[ch], __uint_as_float(ymax[ch]) * coef);
 		const float sc = (gm > 0.0f) ? gm : 1.0f;
 		inv = 1.0f / sc;
 		if (tid == 0) scale[gch] = sc;
@@ -986,6 +1008,7 @@ struct PrepPlan {
 	// four-step path (records too long for the fused kernel)
 	Fft4Plan* f4 = nullptr;     // size Nf / 2
 	DeviceBuffer t4, wpost4;    // transpose buffer; exp(-2 pi i k / Nf), k <= Nf / 4
+	DeviceBuffer chmax;         // per channel of a round: max |x| (float), max |F| (float bits)
 	uint32_t wpost4_nf = 0;
 	~PrepPlan()
 	{
@@ -1005,7 +1028,7 @@ size_t prep_device_bytes(const PrepState& st)
 	const PrepPlan& P = *st.plan;
 	// the C2C plan's work area is taken as one more complex chunk
 	return P.real.capacity() + P.spec.capacity() + 2 * P.z.capacity() + P.tw.capacity() + P.wpost.capacity() +
-			P.t4.capacity() + P.wpost4.capacity() + fft4_device_bytes(P.f4);
+			P.t4.capacity() + P.wpost4.capacity() + P.chmax.capacity() + fft4_device_bytes(P.f4);
 }
 
 static void make_plans(hipfftHandle* r2c, hipfftHandle* c2c, uint32_t nf, uint32_t batch, hipStream_t s)
@@ -1021,14 +1044,14 @@ static void make_plans(hipfftHandle* r2c, hipfftHandle* c2c, uint32_t nf, uint32
 
 template<int VPT>
 static void launch_fir(uint32_t nch, const FrameGeom& g, const float* raw, const float* hpoly, uint32_t J,
-		uint32_t off, float* out, hipStream_t s)
+		uint32_t off, float* out, hipStream_t s, float* xmax = nullptr)
 {
 	const uint32_t right = off / g.up + 2;
 	const size_t lds = (static_cast<size_t>(J) + g.ns + right + static_cast<size_t>(g.up) * J) * sizeof(float);
 	HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(fir_dc_kernel<VPT>),
 			hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
 	hipLaunchKernelGGL(fir_dc_kernel<VPT>, dim3(nch), dim3(kFirThreads), lds, s, raw, g.ns, g.up, g.L, g.nf, g.dz_up,
-			hpoly, J, off, out);
+			hpoly, J, off, out, xmax);
 }
 
 // LDS bytes of the fused kernel for this frame (0 if it cannot run).
@@ -1157,9 +1180,11 @@ static void prepare_fused(PrepState& st, const FrameGeom& g, const float* d_raw,
 // operands from LDS per tap.
 template<int UP, int QPT>
 __global__ void __launch_bounds__(kFirThreads) fir_dc_q_kernel(const float* __restrict__ raw, uint32_t ns,
-		uint32_t L, uint32_t nf, uint32_t dz, const float* __restrict__ hpoly, uint32_t off, float* __restrict__ out)
+		uint32_t L, uint32_t nf, uint32_t dz, const float* __restrict__ hpoly, uint32_t off, float* __restrict__ out,
+		float* __restrict__ xmax)
 {
 	extern __shared__ float smem[];
+	__shared__ float redm[kFirThreads / 64];
 	constexpr uint32_t J = kFirJ;
 	const uint32_t ch = blockIdx.x;
 	const uint32_t tid = threadIdx.x;
@@ -1212,6 +1237,7 @@ __global__ void __launch_bounds__(kFirThreads) fir_dc_q_kernel(const float* __re
 	const float mean = static_cast<float>(red[0] / static_cast<double>(L - dz));
 
 	float* dst = out + static_cast<uint64_t>(ch) * nf;
+	float mx = 0.0f;
 #pragma unroll
 	for (int k = 0; k < QPT; ++k) {
 		const uint32_t q = q_lo + tid + static_cast<uint32_t>(k) * kFirThreads;
@@ -1220,23 +1246,29 @@ __global__ void __launch_bounds__(kFirThreads) fir_dc_q_kernel(const float* __re
 		// a lane's UP outputs are contiguous: 16-byte stores where aligned
 		if (ibase >= 0 && ibase + UP <= static_cast<int>(L) && (ibase & 3) == 0) {
 #pragma unroll
-			for (int h = 0; h < UP; h += 4)
-				*reinterpret_cast<float4*>(dst + ibase + h) =
-						make_float4(y[k][h] - mean, y[k][h + 1] - mean, y[k][h + 2] - mean, y[k][h + 3] - mean);
+			for (int h = 0; h < UP; h += 4) {
+				const float4 o = make_float4(y[k][h] - mean, y[k][h + 1] - mean, y[k][h + 2] - mean, y[k][h + 3] - mean);
+				*reinterpret_cast<float4*>(dst + ibase + h) = o;
+				mx = fmaxf(mx, fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fmaxf(fabsf(o.z), fabsf(o.w))));
+			}
 		} else {
 #pragma unroll
 			for (int ph = 0; ph < UP; ++ph) {
 				const int i = ibase + ph;
-				if (i >= 0 && i < static_cast<int>(L)) dst[i] = y[k][ph] - mean;
+				if (i >= 0 && i < static_cast<int>(L)) {
+					dst[i] = y[k][ph] - mean;
+					mx = fmaxf(mx, fabsf(y[k][ph] - mean));
+				}
 			}
 		}
 	}
 	for (uint32_t i = L + tid; i < nf; i += kFirThreads) dst[i] = 0.0f;
+	if (xmax) store_channel_max(mx, xmax + ch, redm);
 }
 
 // FIR x U + DC removal of n channels into rows of Nf floats (zero padded).
 static void launch_fir_any(uint32_t n, const FrameGeom& g, const float* raw, const float* hpoly, uint32_t J,
-		uint32_t off, float* real, hipStream_t s)
+		uint32_t off, float* real, hipStream_t s, float* xmax = nullptr)
 {
 	if (g.up > 1) {
 		const size_t lds = (static_cast<size_t>(J) + g.ns + off / g.up + 2 + static_cast<size_t>(g.up) * J) * sizeof(float);
@@ -1252,7 +1284,7 @@ static void launch_fir_any(uint32_t n, const FrameGeom& g, const float* raw, con
 		auto go = [&](auto kern) {
 			HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
 					static_cast<int>(lds)));
-			hipLaunchKernelGGL(kern, dim3(n), dim3(kFirThreads), lds, s, raw, g.ns, g.L, g.nf, g.dz_up, hpoly, off, real);
+			hipLaunchKernelGGL(kern, dim3(n), dim3(kFirThreads), lds, s, raw, g.ns, g.L, g.nf, g.dz_up, hpoly, off, real, xmax);
 		};
 		if (g.up == 8) {
 			if (qpt <= 2) go(fir_dc_q_kernel<8, 2>);
@@ -1267,17 +1299,17 @@ static void launch_fir_any(uint32_t n, const FrameGeom& g, const float* raw, con
 		return;
 	}
 	if (g.up == 1) {
-		hipLaunchKernelGGL(copy_dc_kernel, dim3(n), dim3(kFirThreads), 0, s, raw, g.L, g.nf, g.dz_up, real);
+		hipLaunchKernelGGL(copy_dc_kernel, dim3(n), dim3(kFirThreads), 0, s, raw, g.L, g.nf, g.dz_up, real, xmax);
 	} else if (vpt <= 4) {
-		launch_fir<4>(n, g, raw, hpoly, J, off, real, s);
+		launch_fir<4>(n, g, raw, hpoly, J, off, real, s, xmax);
 	} else if (vpt <= 8) {
-		launch_fir<8>(n, g, raw, hpoly, J, off, real, s);
+		launch_fir<8>(n, g, raw, hpoly, J, off, real, s, xmax);
 	} else if (vpt <= 16) {
-		launch_fir<16>(n, g, raw, hpoly, J, off, real, s);
+		launch_fir<16>(n, g, raw, hpoly, J, off, real, s, xmax);
 	} else if (vpt <= 32) {
-		launch_fir<32>(n, g, raw, hpoly, J, off, real, s);
+		launch_fir<32>(n, g, raw, hpoly, J, off, real, s, xmax);
 	} else if (vpt <= 64) {
-		launch_fir<64>(n, g, raw, hpoly, J, off, real, s);
+		launch_fir<64>(n, g, raw, hpoly, J, off, real, s, xmax);
 	} else {
 		MISTA_THROW(MISTA_E_INVALID_PARAM, "signal length " << g.L << " exceeds 65536");
 	}
@@ -1336,27 +1368,32 @@ static void prepare_four_step(PrepState& st, const FrameGeom& g, const float* d_
 	float* real = static_cast<float*>(P.real.reserve(static_cast<size_t>(chunk) * g.nf * sizeof(float)));
 	float2* z = static_cast<float2*>(P.z.reserve(static_cast<size_t>(chunk) * M * sizeof(float2)));
 	float2* t = static_cast<float2*>(P.t4.reserve(static_cast<size_t>(chunk) * M * sizeof(float2)));
+	// FP16/BF16: each channel's largest |x| and |y|, from the FIR and the
+	// inverse transform, for the scale (the store then reads each row once)
+	const bool scaled = g.storage != MISTA_STORAGE_F32;
+	float* xmax = scaled ? static_cast<float*>(P.chmax.reserve(2 * static_cast<size_t>(chunk) * sizeof(float))) : nullptr;
+	uint32_t* ymax = scaled ? reinterpret_cast<uint32_t*>(xmax + chunk) : nullptr;
 	for (uint32_t c0 = 0; c0 < nch; c0 += chunk) {
 		const uint32_t n = std::min(chunk, nch - c0);
 		if (st.before_chunk) st.before_chunk(c0, n);
-		launch_fir_any(n, g, d_raw + static_cast<uint64_t>(c0) * g.ns, d_filter_poly, J, off, real, s);
+		launch_fir_any(n, g, d_raw + static_cast<uint64_t>(c0) * g.ns, d_filter_poly, J, off, real, s, xmax);
 		fft4_exec(*P.f4, reinterpret_cast<const float2*>(real), t, z, n, s);
 		hipLaunchKernelGGL(hilbert_pairs_kernel, dim3((M / 2 + 1 + 255) / 256, n), dim3(256), 0, s, z, M,
-				P.wpost4.as<const float2>());
+				P.wpost4.as<const float2>(), ymax);
 		HIP_CHECK(hipGetLastError());
-		fft4_exec(*P.f4, z, t, z, n, s);
+		fft4_exec(*P.f4, z, t, z, n, s, ymax, g.L / 2, (g.L & 1u) != 0);
 		switch (g.storage) {
 		case MISTA_STORAGE_F32:
 			hipLaunchKernelGGL(store_xy_kernel<MISTA_STORAGE_F32>, dim3(n), dim3(256), 0, s, real, z, g.nf, g.L, d_tensor,
-					d_scale, static_cast<uint64_t>(c0));
+					d_scale, static_cast<uint64_t>(c0), xmax, ymax);
 			break;
 		case MISTA_STORAGE_F16:
 			hipLaunchKernelGGL(store_xy_kernel<MISTA_STORAGE_F16>, dim3(n), dim3(256), 0, s, real, z, g.nf, g.L, d_tensor,
-					d_scale, static_cast<uint64_t>(c0));
+					d_scale, static_cast<uint64_t>(c0), xmax, ymax);
 			break;
 		default:
 			hipLaunchKernelGGL(store_xy_kernel<MISTA_STORAGE_BF16>, dim3(n), dim3(256), 0, s, real, z, g.nf, g.L, d_tensor,
-					d_scale, static_cast<uint64_t>(c0));
+					d_scale, static_cast<uint64_t>(c0), xmax, ymax);
 			break;
 		}
 		HIP_CHECK(hipGetLastError());
