@@ -361,7 +361,7 @@ def serve_mode(args):
                       "storage": "f16", "data": "synthetic",
                       "shapes": [{"elements": wl.num_elements, "samples": wl.num_samples, "grid": [wl.n_x, wl.n_z],
                                   "coherence": wl.cf, "device_ms_alone": round(alone_ms[k], 3),
-                                  "das_kernel": "gather" if tiles[k] < 0 else f"tiled (shape {tiles[k]})"}
+                                  "das_kernel": {-1: "gather", -2: "rx-parallel"}.get(tiles[k], f"tiled (shape {tiles[k]})")}
                                  for k, (wl, _) in enumerate(catalog)],
                       "engine_counters": final, "rates": results}), flush=True)
 
@@ -552,7 +552,7 @@ def tile_name(ran):
     size, ring buffers and LDS budget)."""
     p0, p1 = ran
     if p0 < 0:
-        return "gather kernel"
+        return "gather kernel" if p0 == -1 else "rx-parallel (per-rx sums, then statistics; no tile fits)"
     shape = {2: "8x32", 3: "32x8", 4: "8x8x4", 5: "16x32", 6: "32x16", 7: "8x8x8"}.get(p0, "16x16")
     tc, nb, kb = p1 & 0xFF, (p1 >> 8) & 0xFF, (p1 >> 16) & 0xFF
     budget = f"{kb} KiB budget" if kb else "smallest fitting budget"

@@ -252,8 +252,10 @@ int mista_graph_stats(const mista_handle* h, uint64_t* captures, uint64_t* launc
 int mista_tuned_kernel(const mista_handle* h, int* tile_param0, int* tile_param1);
 
 /* The configuration of the last DAS launch: tile shape (-1 when the gather
- * kernel ran) and pairs per chunk | ring buffers << 8 | LDS budget in KiB
- * << 16 | 64-transmit blocks << 24.  No counterpart in the reference. */
+ * kernel ran, -2 the rx-parallel form that runs where no tile fits: per-rx
+ * sums a lane per (pixel, rx), then the statistics a lane per pixel) and
+ * pairs per chunk | ring buffers << 8 | LDS budget in KiB << 16.  No
+ * counterpart in the reference. */
 int mista_last_kernel(const mista_handle* h, int* tile_param0, int* tile_param1);
 
 /*

@@ -14,6 +14,8 @@ pytestmark = pytest.mark.gpu
 
 from parity import TOL, check_reference_oracle, check_table_oracle, peak_err, say
 
+MM = 1e-3
+
 
 def torch_cuda():
     import torch
@@ -338,8 +340,9 @@ def test_tiled_kernel_over_128_transmits(elements, cf):
     two are bit-identical (without one the gather kernel sums
     transmit-outer, like the reference's no-CF loop, and the two agree to
     the tolerance).  Both 512-lane tile shapes give the same image; a
-    256-lane tile is refused and the gather kernel runs; the frame matches
-    the oracle."""
+    256-lane tile is refused and the rx-parallel form runs (per-rx sums a
+    lane per (pixel, rx), then the statistics in rx order), equal to the
+    tiles bit for bit; the frame matches the oracle."""
     import torch
     wl = workloads.tiny(num_elements=elements, num_samples=2048, upsampling=2, cf=cf, n_x=24, n_z=32)
     echoes = wl.echoes_cpu()
@@ -355,8 +358,9 @@ def test_tiled_kernel_over_128_transmits(elements, cf):
         p.das_device(dv, df)
         torch.cuda.synchronize()
         out[p0] = (dv.cpu().numpy().reshape(wl.n_x, wl.n_z), df.cpu().numpy().reshape(wl.n_x, wl.n_z))
-        # 512-lane shapes run tiled; the 256-lane one falls back to the gather kernel
-        assert (p.last_kernel()[0] >= 0) == (p0 in (5, 6)), (p0, p.last_kernel())
+        # 512-lane shapes run tiled; the 256-lane one falls back to the rx-parallel form
+        expect = p0 if p0 in (5, 6) else (-1 if variant == 1 else -2)
+        assert p.last_kernel()[0] == expect, (p0, p.last_kernel())
     p.set_kernel(0)
     v_a, f_a = p.process_echoes(echoes)
     tuned = p.tuned_kernel()
@@ -366,7 +370,7 @@ def test_tiled_kernel_over_128_transmits(elements, cf):
     say("autotuner chose", tuned)
     assert tuned[0] in (5, 6)
     assert np.array_equal(out[5][0], out[6][0]) and np.array_equal(out[5][1], out[6][1])
-    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[1][0], out[5][0]) and np.array_equal(out[1][1], out[5][1])
     assert np.array_equal(v_a.reshape(wl.n_x, wl.n_z), out[5][0])
     if cf == "none":
         assert peak_err(out[5][0], out[0][0]) <= TOL
@@ -379,6 +383,46 @@ def test_tiled_kernel_over_128_transmits(elements, cf):
                        f"over 128 transmits {elements} {cf}")
     check_reference_oracle(out[5][0], out[5][1], ocfg, a32, xz, d_gpu, wl.rx_apodization(), wl.tx_apodization(),
                            f"over 128 transmits {elements} {cf}")
+
+
+@pytest.mark.parametrize("cf,storage", [("phase_coherence_factor", "f32"), ("sign_coherence_factor", "f16"),
+                                        ("none", "f32")])
+def test_rx_parallel_where_no_tile_fits(cf, storage):
+    """A grid so coarse (24 x 24 points over 20 x 55 mm) that a tile's
+    window passes 64 KiB: no tile shape fits, and the automatic variant runs
+    the rx-parallel form (per-rx sums a lane per (pixel, rx), the statistics
+    a lane per pixel in rx order).  Its per-rx sums are the tiled kernel's fma
+    chain, so with a coherence factor it equals the gather kernel bit for
+    bit (without one the gather kernel sums transmit-outer: within the
+    tolerance); the frame matches the oracle on the GPU's own tensor and
+    delay table."""
+    import torch
+    wl = workloads.Workload2D(name="coarse", num_elements=64, num_samples=4096, upsampling=8, cf=cf,
+                              grid_x=(-10 * MM, 10 * MM), grid_z=(5 * MM, 60 * MM), n_x=24, n_z=24, points=6,
+                              point_x=(-8 * MM, 8 * MM), point_z=(10 * MM, 50 * MM), seed_points=77, seed_noise=78)
+    say("no tile fits", cf, storage)
+    p = make_proc(wl, storage=storage)
+    p.process_echoes(wl.echoes_cpu())
+    n = wl.n_x * wl.n_z
+    out = {}
+    for variant in (0, 1):
+        p.set_kernel(variant)
+        dv = torch.empty(n, dtype=torch.float32, device="cuda")
+        df = torch.empty(n, dtype=torch.float32, device="cuda")
+        p.das_device(dv, df)
+        torch.cuda.synchronize()
+        out[variant] = (dv.cpu().numpy(), df.cpu().numpy())
+        assert p.last_kernel()[0] == (-2 if variant == 0 else -1), (variant, p.last_kernel())
+    a_gpu = p.download_analytic()
+    d_gpu = p.download_delays()
+    p.close()
+    if cf == "none":
+        assert peak_err(out[0][0], out[1][0]) <= TOL
+    else:
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    check_table_oracle(out[0][0], out[0][1], oracle.cfg_for(wl), a_gpu, d_gpu, wl.rx_apodization(),
+                       wl.tx_apodization(), f"rx-parallel {cf} {storage}",
+                       branch_rel=1e-6 if storage != "f32" else 0.0)
 
 
 @pytest.fixture(scope="module")

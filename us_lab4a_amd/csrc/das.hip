@@ -9,9 +9,12 @@
 // w[rx][tx] folds rx apodization, optional tx apodization and the per-channel
 // FP16/BF16 storage scale.
 //
-// Two kernels:
+// Kernels:
 //  das_gather_kernel  one pixel per lane, global gathers (reference shape;
-//                     fallback for any size).
+//                     the explicit gather variant).
+//  das_rx_sums_kernel + das_rx_stats_kernel  the fallback where no tile
+//                     fits: a lane per (pixel, rx) sums over the transmits,
+//                     then a lane per pixel folds the per-rx sums in rx order.
 //  das_tile_kernel    a workgroup owns a pixel tile for the whole frame; the
 //                     tx delays of each lane's pixel stay in registers; for
 //                     every (rx, tx) pair the tile's signal window is copied
@@ -98,6 +101,68 @@ __global__ void __launch_bounds__(256) das_gather_kernel(const DasArgs a)
 			st.add_rx(ar, ai, r == 0);
 		}
 		st.write(a, p);
+	}
+}
+
+// The per-rx sums of the tiled kernel, a lane per (pixel, rx): lanes of a
+// workgroup take consecutive pixels of one rx (grid y), so the delay-table
+// and partial-sum accesses are coalesced and the weights uniform.  The sum
+// over the transmits is the tiled kernel's fma chain in transmit order (an
+// excluded position adds nothing, where the tiled kernel adds it with weight
+// 0); out to partial[rx][pixel].  With far more lanes than pixels this keeps
+// the GPU busy on grids too coarse for any tile (a window past 64 KiB) or too
+// small to fill it one pixel per lane.
+template<int STORAGE>
+__global__ void __launch_bounds__(256) das_rx_sums_kernel(const DasArgs a)
+{
+	const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint32_t r = blockIdx.y;
+	if (p >= a.npix) return;
+	const float* dcol = a.delay + p;
+	const float lim = static_cast<float>(a.L - 1u);
+	const float drx = dcol[static_cast<uint64_t>(r) * a.npix];
+	const float* wrow = a.w + static_cast<uint64_t>(r) * a.w_stride;
+	float ar = 0.0f, ai = 0.0f;
+	for (uint32_t t = 0; t < a.ntx; ++t) {
+		const float pos = (a.offset + dcol[static_cast<uint64_t>(a.first_tx + t) * a.npix]) + drx;
+		if (pos >= 0.0f && pos < lim) {
+			const uint32_t i = static_cast<uint32_t>(pos);
+			const float k = pos - static_cast<float>(i);
+			const uint64_t ch = (static_cast<uint64_t>(t) * a.nrx + r) * a.L + i;
+			const float2 v0 = load_sample<STORAGE>(a.tensor, ch);
+			const float2 v1 = load_sample<STORAGE>(a.tensor, ch + 1);
+			const float w = wrow[t];
+			ar = fmaf(fmaf(k, v1.x - v0.x, v0.x), w, ar);
+			ai = fmaf(fmaf(k, v1.y - v0.y, v0.y), w, ai);
+		}
+	}
+	a.partial[static_cast<uint64_t>(r) * a.npix + p] = make_float2(ar, ai);
+}
+
+// A lane per pixel: the per-rx sums in rx order through the same statistics
+// as both other kernels (so the images equal the tiled kernel's).
+template<int CF>
+__global__ void __launch_bounds__(256) das_rx_stats_kernel(const DasArgs a)
+{
+	const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (p >= a.npix) return;
+	PixelStats<CF> st;
+	for (uint32_t r = 0; r < a.nrx; ++r) {
+		const float2 v = a.partial[static_cast<uint64_t>(r) * a.npix + p];
+		st.add_rx(v.x, v.y, r == 0);
+	}
+	st.write(a, p);
+}
+
+template<int STORAGE>
+void launch_rx_parallel(const FrameGeom& g, const DasArgs& a, hipStream_t s)
+{
+	const uint32_t nb = static_cast<uint32_t>((g.npix + 255) / 256);
+	hipLaunchKernelGGL(das_rx_sums_kernel<STORAGE>, dim3(nb, g.nrx), dim3(256), 0, s, a);
+	switch (g.cf_kind) {
+	case MISTA_CF_NONE: hipLaunchKernelGGL(das_rx_stats_kernel<MISTA_CF_NONE>, dim3(nb), dim3(256), 0, s, a); break;
+	case MISTA_CF_PCF: hipLaunchKernelGGL(das_rx_stats_kernel<MISTA_CF_PCF>, dim3(nb), dim3(256), 0, s, a); break;
+	default: hipLaunchKernelGGL(das_rx_stats_kernel<MISTA_CF_SCF>, dim3(nb), dim3(256), 0, s, a); break;
 	}
 }
 
@@ -721,8 +786,26 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 			}
 		}
 	}
-	scratch.last_p0 = -1;
 	scratch.last_p1 = 0;
+	// no tile fits (or none was asked for): the rx-parallel form unless the
+	// gather kernel was asked for or the per-rx sums would pass 4 GiB
+	const size_t pb = static_cast<size_t>(g.nrx) * g.npix * sizeof(float2);   // partial[rx][pixel]
+	if (t.variant != 1 && pb > 0 && pb <= (static_cast<size_t>(4) << 30)) {
+		DasArgs b = a;
+		b.partial = static_cast<float2*>(scratch.partials.reserve(pb));
+		if (!b.partial || scratch.partials.capacity() < pb)
+			MISTA_THROW(MISTA_E_NO_MEMORY, "no buffer for the per-rx sums (" << pb << " bytes)");
+		scratch.last_p0 = -2;
+		switch (g.storage) {
+		case MISTA_STORAGE_F32: launch_rx_parallel<MISTA_STORAGE_F32>(g, b, s); break;
+		case MISTA_STORAGE_F16: launch_rx_parallel<MISTA_STORAGE_F16>(g, b, s); break;
+		case kStorageReal: launch_rx_parallel<kStorageReal>(g, b, s); break;
+		default: launch_rx_parallel<MISTA_STORAGE_BF16>(g, b, s); break;
+		}
+		HIP_CHECK(hipGetLastError());
+		return;
+	}
+	scratch.last_p0 = -1;
 	switch (g.storage) {
 	case MISTA_STORAGE_F32: launch_gather<MISTA_STORAGE_F32>(g, a, s); break;
 	case MISTA_STORAGE_F16: launch_gather<MISTA_STORAGE_F16>(g, a, s); break;
