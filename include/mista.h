@@ -341,7 +341,7 @@ typedef struct mista_job {
 
 typedef struct mista_job_stats {
 	int32_t status;              /* MISTA_OK or the job's error code            */
-	int32_t das_tile;            /* tile shape of its DAS launch (-1: gather)   */
+	int32_t das_tile;            /* tile shape of its DAS launch (-1: gather, -2: rx-parallel) */
 	double queued_ms;            /* submit -> admitted                          */
 	double latency_ms;           /* submit -> outputs ready (seen by the loop)  */
 	double device_ms;            /* job's work on its stream (HIP events)       */
