@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle leg (baseline + FP64 errors)")
     ap.add_argument("--no-headline", action="store_true", help="skip the FP16 batched comparison")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-reps", type=int, default=5, help="timed repetitions of the CPU sample (after 1 warm-up)")
+    ap.add_argument("--cpu-reps", type=int, default=10, help="timed repetitions of the CPU sample (after 1 warm-up)")
     ap.add_argument("--mode", default="frame", choices=["frame", "batch", "serve"],
                     help="frame: the headline line; batch: FP16 frames through the serving engine vs plain "
                          "FP32 frames; serve: Poisson arrivals of a mixed-shape job stream")
