@@ -425,6 +425,44 @@ def test_rx_parallel_where_no_tile_fits(cf, storage):
                        branch_rel=1e-6 if storage != "f32" else 0.0)
 
 
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_tiled_sixteen_pair_chunks(storage):
+    """512-lane 16x32 tiles over a 128-transmit block with 16 pairs per chunk
+    (two windows per wave per chunk, half the barriers of 8): the same
+    per-rx fma chains, so the images equal the 8-pair chunks' and the gather
+    kernel's bit for bit (PCF).  Pixel pitch as the 128-element frame's
+    (0.074 x 0.059 mm), so the windows fit two 16-pair buffers."""
+    import torch
+    wl = workloads.Workload2D(name="tc16", num_elements=128, num_samples=2048, upsampling=8,
+                              cf="phase_coherence_factor", grid_x=(-4.75 * MM, 4.75 * MM), grid_z=(5 * MM, 20 * MM),
+                              n_x=128, n_z=256, points=12, point_x=(-4 * MM, 4 * MM), point_z=(6 * MM, 19 * MM),
+                              seed_points=91, seed_noise=92)
+    say("16 pairs per chunk", storage)
+    p = make_proc(wl, storage=storage)
+    d = torch.empty((wl.num_tx, wl.num_elements, wl.num_samples), dtype=torch.float32, device="cuda")
+    wl.echoes_gpu(d)
+    torch.cuda.synchronize()
+    p.prepare_signals_device(d, wl.num_samples)
+    torch.cuda.synchronize()
+    n = wl.n_x * wl.n_z
+    out = {}
+    for name, variant, p0, p1 in (("gather", 1, 0, 0), ("8 pairs", 2, 5, 8 | (150 << 16)),
+                                  ("16 pairs", 2, 5, 16 | (150 << 16))):
+        p.set_kernel(variant, p0, p1)
+        dv = torch.empty(n, dtype=torch.float32, device="cuda")
+        df = torch.empty(n, dtype=torch.float32, device="cuda")
+        p.das_device(dv, df)
+        torch.cuda.synchronize()
+        out[name] = (dv.cpu().numpy(), df.cpu().numpy())
+        ran = p.last_kernel()
+        if variant == 2:
+            assert ran[0] == 5 and (ran[1] & 0xFF) == (p1 & 0xFF), (name, ran)
+    p.close()
+    for name in ("8 pairs", "16 pairs"):
+        assert np.array_equal(out[name][0], out["gather"][0]), name
+        assert np.array_equal(out[name][1], out["gather"][1]), name
+
+
 @pytest.fixture(scope="module")
 def graded_fp64():
     """The 128-element PCF frame's echoes (GPU-synthesised) and the FP64

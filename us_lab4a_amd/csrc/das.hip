@@ -377,9 +377,10 @@ bool choose_chunking(const FrameGeom& g, uint32_t nt, int p1, uint32_t S, uint32
 			while (nb > 2 && (nb - 2) * per > 63) --nb;
 			return (nb >= 2 && 2 * per <= 63) ? nb : 0u;
 		};
-		for (uint32_t c : {8u, 4u}) {
+		for (uint32_t c : {16u, 8u, 4u}) {
 			if (want_tc && c != want_tc) continue;
-			if (nt == 512 && c != 8) continue;
+			if (c == 16 && (nt != 512 || want_tc != 16 || g.ntx_pad != 128)) continue;   // 16 pairs: asked for, 512-lane tiles, 128-transmit blocks
+			if (nt == 512 && c == 4) continue;
 			const uint32_t nb = fit(c);
 			if (nb && (nb - 1) * c > (nbuf ? (nbuf - 1) * tc : 0u)) { tc = c; nbuf = nb; }
 		}
@@ -746,6 +747,10 @@ void launch_das(const FrameGeom& g, const DasTuning& t, DasScratch& scratch, con
 					std::vector<int> cands;
 					for (int kb : kbs)
 						for (int tc : {8, 4}) cands.push_back(tc | (kb << 16));
+					// 512-lane tiles over 128-transmit blocks: also 16 pairs per
+					// chunk in the whole LDS (half the barriers; windows of up to
+					// 4 slots, FP16/BF16)
+					if (p0 >= 5 && g.ntx_pad == 128) cands.push_back(16 | (kbs[1] << 16));
 					cands.push_back(0);
 					for (const int p1 : cands) {
 						if (bp0 >= 0 && best > kTuneLimitMs) {

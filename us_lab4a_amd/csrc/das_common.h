@@ -461,7 +461,7 @@ __device__ __forceinline__ void das_tile(const TileArgs& A, const uint32_t tile,
 	// LDS reads issued together: a whole chunk's where the register budget
 	// allows it (two waves per SIMD), four with up to 64 transmit delays in
 	// registers, else one at a time (128 registers hold transmit delays)
-	constexpr int BATCH = NTHREADS >= 512 ? TC : (NTXMAX <= 64 ? 4 : 1);
+	constexpr int BATCH = NTHREADS >= 512 ? (TC > 8 ? 8 : TC) : (NTXMAX <= 64 ? 4 : 1);
 	static_assert(BATCH == 1 || TC % BATCH == 0, "whole read groups per chunk");
 
 	const DasArgs& a = A.d;
@@ -818,6 +818,12 @@ void launch_tile_t(const TileBatch& B, uint32_t nt, uint32_t tc, hipStream_t s)
 		if (A.ntx_pad > NTXMAX) {   // several transmit blocks: 512-lane tiles only
 			if (nt != 512) MISTA_THROW(MISTA_E_INVALID_STATE, "tile of " << nt << " lanes for " << A.ntx_pad << " transmits");
 			MISTA_GO(8, 512, true);
+			return;
+		}
+	}
+	if constexpr (NTXMAX == 128) {
+		if (nt == 512 && tc == 16) {
+			MISTA_GO(16, 512, false);
 			return;
 		}
 	}
