@@ -230,7 +230,10 @@ int mista_download_delays_at(mista_handle* h, const uint64_t* points, uint64_t n
 int mista_query(const mista_handle* h, uint32_t* signal_length, uint32_t* fft_size, uint64_t* num_points);
 
 /* Select the DAS kernel variant (0 = automatic/tuned, 1 = reference-shaped
- * one-pixel-per-lane gather, 2 = LDS-tiled).  Tile parameters 0 = default. */
+ * one-pixel-per-lane gather, 2 = LDS-tiled).  Tile parameters 0 = default;
+ * tile_param1 = pairs per chunk (4, 8, or 16 on 512-lane tiles over 128
+ * transmits) | ring buffers << 8 | LDS budget in KiB << 16.  Where the
+ * variant's tiles do not fit, the rx-parallel form runs instead. */
 int mista_set_kernel(mista_handle* h, int variant, int tile_param0, int tile_param1);
 
 /* Frame graphs for mista_process_device (no counterpart in the reference:
@@ -247,7 +250,7 @@ int mista_graph_stats(const mista_handle* h, uint64_t* captures, uint64_t* launc
 
 /* Tile parameters the automatic variant chose on its first frame of the
  * current geometry (it times each tile shape and LDS budget once); -1 in
- * tile_param0 when nothing was tuned or the gather kernel runs.  No
+ * tile_param0 when nothing was tuned or no tile fits.  No
  * counterpart in the reference (its kernels have fixed launch shapes). */
 int mista_tuned_kernel(const mista_handle* h, int* tile_param0, int* tile_param1);
 

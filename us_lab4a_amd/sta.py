@@ -170,14 +170,15 @@ class HipSTAProcessor:
         return a.value, b.value
 
     def last_kernel(self):
-        """(tile shape, chunking) of the last DAS launch; shape -1: gather."""
+        """(tile shape, chunking) of the last DAS launch; shape -1: gather
+        kernel, -2: rx-parallel form (no tile fits)."""
         a, b = ctypes.c_int32(0), ctypes.c_int32(0)
         check(lib().mista_last_kernel(self._h, ctypes.byref(a), ctypes.byref(b)), self._h)
         return a.value, b.value
 
     def tuned_kernel(self):
         """(tile shape, chunking) the automatic variant chose; (-1, 0) when
-        nothing was tuned or the gather kernel runs."""
+        nothing was tuned or no tile fits."""
         a, b = ctypes.c_int32(0), ctypes.c_int32(0)
         check(lib().mista_tuned_kernel(self._h, ctypes.byref(a), ctypes.byref(b)), self._h)
         return a.value, b.value
