@@ -379,13 +379,15 @@ __device__ __forceinline__ void lds_dma16_slots(const void* g, uint32_t lds, uin
 #define MISTA_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
 __device__ __forceinline__ void wait_vmcnt_at_most(uint32_t n)
 {
-	if (n >= 32) MISTA_VMCNT(32);
-	else if (n >= 16) MISTA_VMCNT(16);
-	else if (n >= 8) MISTA_VMCNT(8);
-	else if (n >= 4) MISTA_VMCNT(4);
-	else if (n >= 2) MISTA_VMCNT(2);
-	else if (n >= 1) MISTA_VMCNT(1);
-	else MISTA_VMCNT(0);
+	// small counts first: with two ring buffers (the common case) n is 0,
+	// decided by one compare
+	if (n < 1) MISTA_VMCNT(0);
+	else if (n < 2) MISTA_VMCNT(1);
+	else if (n < 4) MISTA_VMCNT(2);
+	else if (n < 8) MISTA_VMCNT(4);
+	else if (n < 16) MISTA_VMCNT(8);
+	else if (n < 32) MISTA_VMCNT(16);
+	else MISTA_VMCNT(32);
 }
 
 // N (4 or 8) LDS reads of two consecutive entries each (byte addresses
