@@ -377,17 +377,48 @@ __device__ __forceinline__ void lds_dma16_slots(const void* g, uint32_t lds, uin
 // as s_waitcnt vmcnt(n), with 7 immediates instead of 64.  The expcnt and
 // lgkmcnt fields stay at their maxima (no wait on them).
 #define MISTA_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
+// Written as one asm block of compares and branches: as C++ the if-chain
+// comes out restructured into flag registers and several branch blocks that
+// every chunk walks through (about 15 scalar instructions for n = 0 instead of
+// 3).  Small counts first: with two ring buffers (the common case) n is 0.
 __device__ __forceinline__ void wait_vmcnt_at_most(uint32_t n)
 {
-	// small counts first: with two ring buffers (the common case) n is 0,
-	// decided by one compare
-	if (n < 1) MISTA_VMCNT(0);
-	else if (n < 2) MISTA_VMCNT(1);
-	else if (n < 4) MISTA_VMCNT(2);
-	else if (n < 8) MISTA_VMCNT(4);
-	else if (n < 16) MISTA_VMCNT(8);
-	else if (n < 32) MISTA_VMCNT(16);
-	else MISTA_VMCNT(32);
+	asm volatile(
+			"s_cmp_lt_u32 %0, 1\n\t"
+			"s_cbranch_scc0 .Lvw1_%=\n\t"
+			"s_waitcnt vmcnt(0)\n\t"
+			"s_branch .Lvwe_%=\n"
+			".Lvw1_%=:\n\t"
+			"s_cmp_lt_u32 %0, 2\n\t"
+			"s_cbranch_scc0 .Lvw2_%=\n\t"
+			"s_waitcnt vmcnt(1)\n\t"
+			"s_branch .Lvwe_%=\n"
+			".Lvw2_%=:\n\t"
+			"s_cmp_lt_u32 %0, 4\n\t"
+			"s_cbranch_scc0 .Lvw4_%=\n\t"
+			"s_waitcnt vmcnt(2)\n\t"
+			"s_branch .Lvwe_%=\n"
+			".Lvw4_%=:\n\t"
+			"s_cmp_lt_u32 %0, 8\n\t"
+			"s_cbranch_scc0 .Lvw8_%=\n\t"
+			"s_waitcnt vmcnt(4)\n\t"
+			"s_branch .Lvwe_%=\n"
+			".Lvw8_%=:\n\t"
+			"s_cmp_lt_u32 %0, 16\n\t"
+			"s_cbranch_scc0 .Lvw16_%=\n\t"
+			"s_waitcnt vmcnt(8)\n\t"
+			"s_branch .Lvwe_%=\n"
+			".Lvw16_%=:\n\t"
+			"s_cmp_lt_u32 %0, 32\n\t"
+			"s_cbranch_scc0 .Lvw32_%=\n\t"
+			"s_waitcnt vmcnt(16)\n\t"
+			"s_branch .Lvwe_%=\n"
+			".Lvw32_%=:\n\t"
+			"s_waitcnt vmcnt(32)\n"
+			".Lvwe_%=:"
+			:
+			: "s"(n)
+			: "scc", "memory");
 }
 
 // N (4 or 8) LDS reads of two consecutive entries each (byte addresses
