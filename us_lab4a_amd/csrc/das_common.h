@@ -421,47 +421,31 @@ __device__ __forceinline__ void wait_vmcnt_at_most(uint32_t n)
 			: "scc", "memory");
 }
 
-// N (4 or 8) LDS reads of two consecutive entries each (byte addresses
-// addr), issued back to back and waited for once, in one asm block: the compiler
-// would otherwise wait for each read before the next.  The outputs are
-// early-clobbered so no address register is reused while a read is in flight.
-#pragma clang diagnostic push
-#pragma clang diagnostic ignored "-Winline-asm"
-#define MISTA_RD2(op, o, a) op " %" #o ", %" #a " offset1:1\n\t"
+// Two consecutive LDS entries as one value of LdsPair's type (4-byte
+// entries as their raw words).
+__device__ __forceinline__ float4 pack_pair(uint64_t a, uint64_t b)
+{
+	const float2 u = __builtin_bit_cast(float2, a), v = __builtin_bit_cast(float2, b);
+	return make_float4(u.x, u.y, v.x, v.y);
+}
+__device__ __forceinline__ uint2 pack_pair(uint32_t a, uint32_t b) { return make_uint2(a, b); }
+
+// N LDS reads of two consecutive entries each (byte addresses addr; one
+// ds_read2 per pair), written as plain loads and all issued before the
+// first result is used: the compiler's wait insertion then waits for read
+// q just before its use (lgkmcnt(N-1-q), the reads complete in order), so
+// the first pairs' accumulation overlaps the later reads' latency.
 template<uint32_t ESZ, int N>
 __device__ __forceinline__ void lds_read_pairs(const uint32_t* addr, typename LdsPair<ESZ>::T* out)
 {
-	static_assert(N == 4 || N == 8, "four or eight reads");
-	if constexpr (N == 4 && ESZ == 8) {
-		asm volatile(MISTA_RD2("ds_read2_b64", 0, 4) MISTA_RD2("ds_read2_b64", 1, 5) MISTA_RD2("ds_read2_b64", 2, 6)
-				MISTA_RD2("ds_read2_b64", 3, 7) "s_waitcnt lgkmcnt(0)"
-				: "=&v"(out[0]), "=&v"(out[1]), "=&v"(out[2]), "=&v"(out[3])
-				: "v"(addr[0]), "v"(addr[1]), "v"(addr[2]), "v"(addr[3]) : "memory");
-	} else if constexpr (N == 4) {
-		asm volatile(MISTA_RD2("ds_read2_b32", 0, 4) MISTA_RD2("ds_read2_b32", 1, 5) MISTA_RD2("ds_read2_b32", 2, 6)
-				MISTA_RD2("ds_read2_b32", 3, 7) "s_waitcnt lgkmcnt(0)"
-				: "=&v"(out[0]), "=&v"(out[1]), "=&v"(out[2]), "=&v"(out[3])
-				: "v"(addr[0]), "v"(addr[1]), "v"(addr[2]), "v"(addr[3]) : "memory");
-	} else if constexpr (ESZ == 8) {
-		asm volatile(MISTA_RD2("ds_read2_b64", 0, 8) MISTA_RD2("ds_read2_b64", 1, 9) MISTA_RD2("ds_read2_b64", 2, 10)
-				MISTA_RD2("ds_read2_b64", 3, 11) MISTA_RD2("ds_read2_b64", 4, 12) MISTA_RD2("ds_read2_b64", 5, 13)
-				MISTA_RD2("ds_read2_b64", 6, 14) MISTA_RD2("ds_read2_b64", 7, 15) "s_waitcnt lgkmcnt(0)"
-				: "=&v"(out[0]), "=&v"(out[1]), "=&v"(out[2]), "=&v"(out[3]), "=&v"(out[4]), "=&v"(out[5]),
-				  "=&v"(out[6]), "=&v"(out[7])
-				: "v"(addr[0]), "v"(addr[1]), "v"(addr[2]), "v"(addr[3]), "v"(addr[4]), "v"(addr[5]), "v"(addr[6]),
-				  "v"(addr[7]) : "memory");
-	} else {
-		asm volatile(MISTA_RD2("ds_read2_b32", 0, 8) MISTA_RD2("ds_read2_b32", 1, 9) MISTA_RD2("ds_read2_b32", 2, 10)
-				MISTA_RD2("ds_read2_b32", 3, 11) MISTA_RD2("ds_read2_b32", 4, 12) MISTA_RD2("ds_read2_b32", 5, 13)
-				MISTA_RD2("ds_read2_b32", 6, 14) MISTA_RD2("ds_read2_b32", 7, 15) "s_waitcnt lgkmcnt(0)"
-				: "=&v"(out[0]), "=&v"(out[1]), "=&v"(out[2]), "=&v"(out[3]), "=&v"(out[4]), "=&v"(out[5]),
-				  "=&v"(out[6]), "=&v"(out[7])
-				: "v"(addr[0]), "v"(addr[1]), "v"(addr[2]), "v"(addr[3]), "v"(addr[4]), "v"(addr[5]), "v"(addr[6]),
-				  "v"(addr[7]) : "memory");
+	using W = typename std::conditional<ESZ == 8, uint64_t, uint32_t>::type;
+	typedef __attribute__((address_space(3))) const W* lds_entry_t;
+#pragma unroll
+	for (int q = 0; q < N; ++q) {
+		const lds_entry_t lp = (lds_entry_t)(size_t)addr[q];
+		out[q] = pack_pair(lp[0], lp[1]);
 	}
 }
-#undef MISTA_RD2
-#pragma clang diagnostic pop
 
 // Chunk i of a tile = pairs (rx r, tx c*TC .. c*TC+TC-1), r = i / CPR,
 // c = i % CPR.  Chunks stream through a ring of nbuf LDS buffers, each
