@@ -456,7 +456,7 @@ __device__ __forceinline__ void lds_read_pairs(const uint32_t* addr, typename Ld
 // entry, so every wave issues the same number of copies per chunk and waits
 // for chunk i alone with s_waitcnt vmcnt.  The chunk's window words and
 // weights are scalar-loaded one chunk ahead; its TC LDS reads are issued
-// together and waited for once.  Chunks in which no pair can reach a record
+// together, each pair waiting for its own.  Chunks in which no pair can reach a record
 // end (the common case) take a path without the range test.
 // Occupancy: 256-lane workgroups target three waves per SIMD (up to 170
 // registers each); a 512-lane workgroup with its large LDS ring holds a CU
