@@ -1,7 +1,7 @@
 """Shared parity checks of the GPU tests against the CPU oracle (test
 infrastructure; imported by tests/test_gpu_*.py only).
 
-Two oracles judge a GPU image:
+Two oracles check a GPU image:
 
 - the reference-equivalent FP32 restatement (`oracle.das`), which computes
   its own FP32 element-to-pixel distances like the reference CPU processor
