@@ -652,6 +652,11 @@ __device__ __forceinline__ void das_tile(const TileArgs& A, const uint32_t tile,
 				else after = (nchunks - 1 - i) * Q + (nrx - 1 - r);
 				wait_vmcnt_at_most(after);         // this wave's copies of chunk i landed
 				__syncthreads();                   // everyone's did; buffer of chunk i-1 is free
+				// Waves still issuing copies, positions and LDS reads go
+				// before waves already accumulating (the reads then start
+				// sooner after the barrier): F32 DAS 9.97 -> 9.94 ms, F16
+				// 6.33 -> 6.26 ms.  Lowered once the chunk's last reads are issued.
+				if constexpr (BATCH > 1) __builtin_amdgcn_s_setprio(2);
 				if (MULTI && c == 0) --pend;
 				if (is_k < nchunks) issue();
 				if (c == 0) drx = *reinterpret_cast<const float*>(smem + bbase + win_bytes + tid * 4);
@@ -711,6 +716,7 @@ __device__ __forceinline__ void das_tile(const TileArgs& A, const uint32_t tile,
 							for (int q = 0; q < BATCH; ++q) pair(g + q, addr[q], kf[q], wv[q]);
 							typename LdsPair<ESZ>::T e[BATCH];
 							lds_read_pairs<ESZ, BATCH>(addr, e);
+							if (g + BATCH >= TC) __builtin_amdgcn_s_setprio(0);
 #pragma unroll
 							for (int q = 0; q < BATCH; ++q) accumulate(Entry<STORAGE>::lerp2(e[q], kf[q]), wv[q]);
 						}
